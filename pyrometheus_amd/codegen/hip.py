@@ -1,0 +1,498 @@
+"""The ``hip`` code-generation target.
+
+``HipCodeGenerator.generate(name, mech)`` lowers the expression semantics of
+``pyrometheus/chem_expr.py`` into one HIP C++ translation unit for gfx950:
+
+* one cell per lane; every mechanism constant (NASA-7 coefficients, Arrhenius
+  parameters, Troe coefficients, third-body efficiencies, stoichiometry) is a
+  compile-time immediate, so nothing is loaded per reaction;
+* species vectors are register arrays indexed only by constants;
+* the kinetics are emitted reaction-major: each reaction's ``k_f``, ``-ln K_c``
+  and rate of progress are computed and immediately folded into the
+  per-species accumulators, so no ``(nr, N)`` intermediate exists;
+* the mechanism-independent kernels and the C ABI come from the hand-written
+  ``csrc/pyro_thermo_kernels.hpp``, which the generated unit includes.
+
+Expression-level correspondences with the reference (all FP64):
+
+=============================  =============================================
+reference                      generated form
+=============================  =============================================
+``nasa7_conditional``          coefficient select on ``T > T_mid``, Horner
+(``chem_expr.py:51-174``)      (strict ``>``: ``T == T_mid`` takes low)
+``rate_coefficient_expr``      ``Ea == 0``: ``A*T^b`` (``b`` integer: exact
+(``chem_expr.py:310-324``)     multiplies; else ``A*exp(b ln T)``);
+                               else ``exp(ln A + b ln T - Ta/T)``
+``third_body_efficiencies``    ``d*sum(C) + sum((eff-d)*C_k)``
+(``chem_expr.py:327-345``)
+Troe / Lindemann               ``Fcent``, ``f1`` with the ``Pr > 0`` guard,
+(``chem_expr.py:348-421``)     ``F = 10^(Fcent/(1+f1^2))``
+``equilibrium_constants_expr`` ``sum_p nu g - sum_r nu g - dnu*c0``;
+(``chem_expr.py:274-302``)     irreversible placeholder ``-0.17364695002734*T``
+``rate_of_progress_expr``      ``k_f*(prod C^nu' - exp(-ln K_c) prod C^nu'')``
+(``chem_expr.py:428-460``)     with the ``orders`` zip quirk kept
+``production_rate_expr``       accumulate ``+/- nu*q`` per species
+=============================  =============================================
+
+The arithmetic is reassociated (Horner, fused multiply-add) relative to the
+numpy source, so results agree with the reference to rounding, not bitwise.
+"""
+
+from __future__ import annotations
+
+import math
+import os
+from typing import Dict, List, Optional, Sequence, Tuple
+
+from ..mechanism import GAS_CONSTANT, ONE_ATM, Mechanism, Reaction
+from . import CodeGenerationOptions, CodeGenerator
+
+ABI_VERSION = 1
+CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                    "csrc")
+
+
+def lit(x: float) -> str:
+    """Exact C++ double literal."""
+    x = float(x)
+    if math.isinf(x):
+        return "(__builtin_inf())" if x > 0 else "(-__builtin_inf())"
+    if math.isnan(x):
+        return "(__builtin_nan(\"\"))"
+    r = repr(x)
+    if "e" not in r and "." not in r and "inf" not in r:
+        r += ".0"
+    return r
+
+
+def _is_small_int(x: float, bound: int = 4) -> bool:
+    return float(x).is_integer() and abs(x) <= bound
+
+
+class _Writer:
+    def __init__(self):
+        self.lines: List[str] = []
+        self.ind = 0
+
+    def __call__(self, s: str = ""):
+        self.lines.append(("  " * self.ind + s) if s else "")
+
+    def block(self, head: str):
+        self(head + " {")
+        self.ind += 1
+
+    def end(self, tail: str = "}"):
+        self.ind -= 1
+        self(tail)
+
+    def text(self) -> str:
+        return "\n".join(self.lines) + "\n"
+
+
+class _Lowering:
+    """Lowers one mechanism; methods return C++ expressions or emit blocks."""
+
+    def __init__(self, mech: Mechanism):
+        self.m = mech
+        self.ns = mech.n_species
+        self.idx = {s.name: i for i, s in enumerate(mech.species)}
+        self.tmids = sorted({s.t_mid for s in mech.species})
+        self.tmid_flag = {t: f"hi{j}" for j, t in enumerate(self.tmids)}
+
+    # --- thermo ---------------------------------------------------------
+    def _flags(self, w: _Writer, T: str = "T"):
+        for t, f in self.tmid_flag.items():
+            w(f"const bool {f} = {T} > {lit(t)};")
+
+    def _sel(self, sp, a_hi: float, a_lo: float) -> str:
+        if a_hi == a_lo:
+            return lit(a_hi)
+        return f"({self.tmid_flag[sp.t_mid]} ? {lit(a_hi)} : {lit(a_lo)})"
+
+    @staticmethod
+    def _horner(coeffs: Sequence[str], x: str) -> str:
+        # coeffs[0] + x*(coeffs[1] + x*(...))
+        expr = coeffs[-1]
+        for c in reversed(coeffs[:-1]):
+            expr = f"fma({x}, {expr}, {c})"
+        return expr
+
+    def emit_species_fn(self, w: _Writer, fname: str, args: str, gen):
+        w(f"__device__ __forceinline__ void {fname}({args}, double (&o)[NS]) {{")
+        w.ind += 1
+        self._flags(w)
+        for k, sp in enumerate(self.m.species):
+            w(f"o[{k}] = {gen(sp)};  // {sp.name}")
+        w.ind -= 1
+        w("}")
+
+    def cp_expr(self, sp) -> str:
+        hi, lo = sp.high, sp.low
+        cs = [self._sel(sp, hi[j], lo[j]) for j in range(5)]
+        return self._horner(cs, "T")
+
+    def h_expr(self, sp) -> str:
+        hi, lo = sp.high, sp.low
+        cs = [self._sel(sp, hi[0], lo[0])]
+        for j, d in ((1, 2), (2, 3), (3, 4), (4, 5)):
+            cs.append(self._sel(sp, hi[j] / d, lo[j] / d))
+        poly = self._horner(cs, "T")
+        return f"fma({self._sel(sp, hi[5], lo[5])}, invT, {poly})"
+
+    def s_expr(self, sp) -> str:
+        hi, lo = sp.high, sp.low
+        cs = ["0.0", self._sel(sp, hi[1], lo[1])]
+        for j, d in ((2, 2), (3, 3), (4, 4)):
+            cs.append(self._sel(sp, hi[j] / d, lo[j] / d))
+        # T*(a1 + T*(a2/2 + ...)) + a0*lnT + a6
+        inner = self._horner(cs[1:], "T")
+        return (f"fma({self._sel(sp, hi[0], lo[0])}, lnT, "
+                f"fma(T, {inner}, {self._sel(sp, hi[6], lo[6])}))")
+
+    def dcp_expr(self, sp) -> str:
+        hi, lo = sp.high, sp.low
+        cs = [self._sel(sp, hi[1], lo[1]), self._sel(sp, 2 * hi[2], 2 * lo[2]),
+              self._sel(sp, 3 * hi[3], 3 * lo[3]),
+              self._sel(sp, 4 * hi[4], 4 * lo[4])]
+        return self._horner(cs, "T")
+
+    def dh_expr(self, sp) -> str:
+        hi, lo = sp.high, sp.low
+        cs = [self._sel(sp, hi[1] / 2, lo[1] / 2),
+              self._sel(sp, 2 * hi[2] / 3, 2 * lo[2] / 3),
+              self._sel(sp, 3 * hi[3] / 4, 3 * lo[3] / 4),
+              self._sel(sp, 4 * hi[4] / 5, 4 * lo[4] / 5)]
+        return (f"fma({self._sel(sp, -hi[5], -lo[5])}, invT * invT, "
+                f"{self._horner(cs, 'T')})")
+
+    def ds_expr(self, sp) -> str:
+        hi, lo = sp.high, sp.low
+        cs = [self._sel(sp, hi[j], lo[j]) for j in (1, 2, 3, 4)]
+        return f"fma({self._sel(sp, hi[0], lo[0])}, invT, {self._horner(cs, 'T')})"
+
+    # --- kinetics -------------------------------------------------------
+    @staticmethod
+    def arrhenius(rate) -> str:
+        a, b, ta = rate.A, rate.b, rate.t_a
+        if ta == 0:
+            if b == 0:
+                return lit(a)
+            if _is_small_int(b):
+                return f"({lit(a)} * pyro::ipow<{int(b)}>(T))"
+            return f"({lit(a)} * exp({lit(b)} * lnT))"
+        lna = math.log(a)
+        if b == 0:
+            return f"exp(fma({lit(-ta)}, invT, {lit(lna)}))"
+        return f"exp(fma({lit(b)}, lnT, fma({lit(-ta)}, invT, {lit(lna)})))"
+
+    def third_body(self, r: Reaction) -> str:
+        d = r.default_efficiency
+        terms = []
+        for sp, eff in r.efficiencies:
+            coef = eff - d
+            if coef != 0.0:
+                terms.append((coef, self.idx[sp]))
+        expr = "0.0" if d == 0 else ("Ctot" if d == 1.0 else f"({lit(d)} * Ctot)")
+        for coef, k in terms:
+            expr = f"fma({lit(coef)}, C[{k}], {expr})"
+        return expr
+
+    def uses_ctot(self) -> bool:
+        return any(r.default_efficiency != 0 for r in self.m.reactions
+                   if r.kind in ("three-body", "falloff"))
+
+    def kf_block(self, w: _Writer, r: Reaction, out: str = "kf"):
+        """Emit statements defining ``out`` = forward rate coefficient with
+        falloff and three-body factors (codegen/python.py:398-473)."""
+        if r.kind == "falloff":
+            w(f"const double khi = {self.arrhenius(r.high_rate)};")
+            w(f"const double klo = {self.arrhenius(r.low_rate)};")
+            w(f"const double pr = ({self.third_body(r)}) * klo / khi;")
+            if r.falloff_type == "Troe":
+                tp = r.troe
+                t1 = f"({lit(1.0 - tp[0])} * exp(T * {lit(-1.0 / tp[1])}))"
+                t2 = f"({lit(tp[0])} * exp(T * {lit(-1.0 / tp[2])}))"
+                s = f"{t1} + {t2}"
+                if len(tp) == 4:
+                    s += f" + exp({lit(-tp[3])} * invT)"
+                w(f"const double fc = log10({s});")
+                w("const double lpr = log10(pr);")
+                w("const double cc = fma(-0.67, fc, -0.4);")
+                w("const double nn = fma(-1.27, fc, 0.75);")
+                w("const double f1 = pr > 0.0 ? (lpr + cc) / fma(-0.14, lpr + cc, nn)"
+                  f" : {lit(-1 / 0.14)};")
+                w("const double fF = exp10(fc / fma(f1, f1, 1.0));")
+                w(f"const double {out} = khi * (fF * pr / (1.0 + pr));")
+            else:
+                w(f"const double {out} = khi * (pr / (1.0 + pr));")
+        elif r.kind == "three-body":
+            w(f"const double {out} = {self.arrhenius(r.rate)} * ({self.third_body(r)});")
+        else:
+            w(f"const double {out} = {self.arrhenius(r.rate)};")
+
+    def _pow_term(self, k: int, nu: float) -> str:
+        if nu == 1.0:
+            return f"C[{k}]"
+        if _is_small_int(nu) and nu > 0:
+            return "(" + " * ".join([f"C[{k}]"] * int(nu)) + ")"
+        return f"pow(C[{k}], {lit(nu)})"
+
+    def reactant_pairs(self, r: Reaction) -> List[Tuple[int, float]]:
+        ind = [self.idx[sp] for sp, _ in r.reactants]
+        if r.orders:
+            nus = [o for _, o in r.orders]  # zip quirk, chem_expr.py:436-448
+        else:
+            nus = [nu for _, nu in r.reactants]
+        return list(zip(ind, nus))
+
+    def prod_expr(self, pairs) -> str:
+        terms = [self._pow_term(k, nu) for k, nu in pairs]
+        return " * ".join(terms) if terms else "1.0"
+
+    def nlk_expr(self, r: Reaction) -> str:
+        def side(d):
+            ts = []
+            for sp, nu in d:
+                g = f"g[{self.idx[sp]}]"
+                ts.append(g if nu == 1.0 else f"{lit(nu)} * {g}")
+            return "(" + " + ".join(ts) + ")" if ts else "0.0"
+        expr = f"{side(r.products)} - {side(r.reactants)}"
+        dnu = sum(nu for _, nu in r.products) - sum(nu for _, nu in r.reactants)
+        if dnu != 0:
+            expr = f"fma({lit(-dnu)}, c0, {expr})"
+        return expr
+
+    def species_updates(self, r: Reaction, q: str) -> List[Tuple[int, float]]:
+        """Net stoichiometric updates: production_rate_expr sums
+        nu'' over products minus nu' over reactants."""
+        upd: Dict[int, float] = {}
+        for sp, nu in r.reactants:
+            upd[self.idx[sp]] = upd.get(self.idx[sp], 0.0) - nu
+        for sp, nu in r.products:
+            upd[self.idx[sp]] = upd.get(self.idx[sp], 0.0) + nu
+        return [(k, v) for k, v in upd.items() if v != 0.0]
+
+
+def _emit_prelude(w: _Writer, L: _Lowering, name: str, digest: str,
+                  wdot_block: int):
+    m = L.m
+    w(f"// Generated by pyrometheus_amd.codegen.hip for mechanism '{name}'")
+    w(f"// source: {m.source}; digest {digest}")
+    w(f"// {m.n_species} species, {m.n_reactions} reactions "
+      f"({m.num_falloff} falloff, {len(m.three_body_reactions)} three-body)")
+    w("// Do not edit: regenerate with HipCodeGenerator.generate().")
+    w('#include "pyro_common.hpp"')
+    w("")
+    w("namespace mech {")
+    w(f"constexpr int NS = {m.n_species};")
+    w(f"constexpr int NR = {m.n_reactions};")
+    w(f"constexpr int NF = {m.num_falloff};")
+    w(f"constexpr int kAbiVersion = {ABI_VERSION};")
+    w(f"constexpr int kWdotBlock = {wdot_block};")
+    w("__device__ constexpr double kW[NS] = {"
+      + ", ".join(lit(x) for x in m.molecular_weights) + "};")
+    w("__device__ constexpr double kInvW[NS] = {"
+      + ", ".join(lit(1.0 / x) for x in m.molecular_weights) + "};")
+    w("using pyro::kGasConstant;")
+    w("using pyro::kOneAtm;")
+    w("")
+
+
+def _emit_thermo(w: _Writer, L: _Lowering):
+    L.emit_species_fn(w, "species_cp_r", "const double T", L.cp_expr)
+    L.emit_species_fn(w, "species_h_rt", "const double T, const double invT",
+                      L.h_expr)
+    L.emit_species_fn(w, "species_s_r", "const double T, const double lnT",
+                      L.s_expr)
+    L.emit_species_fn(w, "species_dcp_r", "const double T", L.dcp_expr)
+    L.emit_species_fn(w, "species_dh_rt", "const double T, const double invT",
+                      L.dh_expr)
+    L.emit_species_fn(w, "species_ds_r", "const double T, const double invT",
+                      L.ds_expr)
+    w("__device__ __forceinline__ void species_g_rt(const double T, "
+      "const double lnT, const double invT, double (&o)[NS]) {")
+    w.ind += 1
+    w("double h[NS], s[NS];")
+    w("species_h_rt(T, invT, h);")
+    w("species_s_r(T, lnT, s);")
+    w("#pragma unroll")
+    w("for (int k = 0; k < NS; ++k) o[k] = h[k] - s[k];")
+    w.ind -= 1
+    w("}")
+    w("")
+
+
+def _kin_prologue(w: _Writer, L: _Lowering, need_g: bool):
+    w("const double lnT = log(T);")
+    w("const double invT = 1.0 / T;")
+    if need_g:
+        w("const double c0 = log(kOneAtm / (kGasConstant * T));")
+        w("double g[NS];")
+        w("species_g_rt(T, lnT, invT, g);")
+    if L.uses_ctot():
+        w("double Ctot = 0.0;")
+        w("#pragma unroll")
+        w("for (int k = 0; k < NS; ++k) Ctot += C[k];")
+
+
+def _emit_wdot(w: _Writer, L: _Lowering):
+    m = L.m
+    w("// fused net production rates (codegen/python.py:475-495)")
+    w("__device__ __forceinline__ void net_production_rates(const double T, "
+      "const double (&C)[NS], double (&w)[NS]) {")
+    w.ind += 1
+    _kin_prologue(w, L, need_g=any(r.reversible for r in m.reactions))
+    w("#pragma unroll")
+    w("for (int k = 0; k < NS; ++k) w[k] = 0.0;")
+    for i, r in enumerate(m.reactions):
+        w(f"{{  // R{i}: {r.equation} [{r.reaction_type}]")
+        w.ind += 1
+        L.kf_block(w, r)
+        rf = L.prod_expr(L.reactant_pairs(r))
+        if r.reversible:
+            rr = L.prod_expr([(L.idx[sp], nu) for sp, nu in r.products])
+            w(f"const double q = kf * ({rf} - exp({L.nlk_expr(r)}) * {rr});")
+        else:
+            w(f"const double q = kf * ({rf});")
+        for k, nu in L.species_updates(r, "q"):
+            if nu == 1.0:
+                w(f"w[{k}] += q;")
+            elif nu == -1.0:
+                w(f"w[{k}] -= q;")
+            else:
+                w(f"w[{k}] = fma({lit(nu)}, q, w[{k}]);")
+        w.ind -= 1
+        w("}")
+    w.ind -= 1
+    w("}")
+    w("")
+
+
+def _emit_reaction_quantities(w: _Writer, L: _Lowering):
+    m = L.m
+    w("// per-reaction k_f, -ln K_c, r_f, r_r, r_net to memory "
+      "(codegen/python.py:360-515)")
+    w("__device__ __forceinline__ void reaction_quantities(const double T, "
+      "const double (&C)[NS], const int64_t i, const int64_t ld, "
+      "double* __restrict__ okf, double* __restrict__ onlk, "
+      "double* __restrict__ orf, double* __restrict__ orr, "
+      "double* __restrict__ onet) {")
+    w.ind += 1
+    _kin_prologue(w, L, need_g=True)
+    for j, r in enumerate(m.reactions):
+        w(f"{{  // R{j}: {r.equation} [{r.reaction_type}]")
+        w.ind += 1
+        L.kf_block(w, r)
+        rf = L.prod_expr(L.reactant_pairs(r))
+        if r.reversible:
+            rr = L.prod_expr([(L.idx[sp], nu) for sp, nu in r.products])
+            w(f"const double nlk = {L.nlk_expr(r)};")
+            w(f"const double pf = {rf};")
+            w(f"const double pb = {rr};")
+            w(f"if (okf) okf[{j} * ld + i] = kf;")
+            w(f"if (onlk) onlk[{j} * ld + i] = nlk;")
+            w(f"if (orf) orf[{j} * ld + i] = kf * pf;")
+            w(f"if (orr) orr[{j} * ld + i] = kf * exp(nlk) * pb;")
+            w(f"if (onet) onet[{j} * ld + i] = kf * (pf - exp(nlk) * pb);")
+        else:
+            w(f"const double pf = {rf};")
+            w(f"if (okf) okf[{j} * ld + i] = kf;")
+            w(f"if (onlk) onlk[{j} * ld + i] = -0.17364695002734 * T;")
+            w(f"if (orf) orf[{j} * ld + i] = kf * pf;")
+            w(f"if (orr) orr[{j} * ld + i] = 0.0;")
+            w(f"if (onet) onet[{j} * ld + i] = kf * pf;")
+        w.ind -= 1
+        w("}")
+    w.ind -= 1
+    w("}")
+    w("")
+
+
+def _emit_creation_destruction(w: _Writer, L: _Lowering):
+    m = L.m
+    w("// creation/destruction split (chem_expr.py:542-567)")
+    w("__device__ __forceinline__ void creation_destruction(const double T, "
+      "const double (&C)[NS], double (&cd)[NS], double (&dd)[NS]) {")
+    w.ind += 1
+    _kin_prologue(w, L, need_g=any(r.reversible for r in m.reactions))
+    w("#pragma unroll")
+    w("for (int k = 0; k < NS; ++k) { cd[k] = 0.0; dd[k] = 0.0; }")
+    for j, r in enumerate(m.reactions):
+        w(f"{{  // R{j}: {r.equation}")
+        w.ind += 1
+        L.kf_block(w, r)
+        w(f"const double qf = kf * ({L.prod_expr(L.reactant_pairs(r))});")
+        if r.reversible:
+            rr = L.prod_expr([(L.idx[sp], nu) for sp, nu in r.products])
+            w(f"const double qr = kf * exp({L.nlk_expr(r)}) * ({rr});")
+        for sp, nu in r.reactants:
+            k = L.idx[sp]
+            w(f"dd[{k}] = fma({lit(nu)}, qf, dd[{k}]);")
+            if r.reversible:
+                w(f"cd[{k}] = fma({lit(nu)}, qr, cd[{k}]);")
+        for sp, nu in r.products:
+            k = L.idx[sp]
+            w(f"cd[{k}] = fma({lit(nu)}, qf, cd[{k}]);")
+            if r.reversible:
+                w(f"dd[{k}] = fma({lit(nu)}, qr, dd[{k}]);")
+        w.ind -= 1
+        w("}")
+    w.ind -= 1
+    w("}")
+    w("")
+
+
+class HipCodeGenerator(CodeGenerator):
+    """``hip`` target: mechanism IR -> HIP C++ translation unit."""
+
+    @staticmethod
+    def get_name() -> str:
+        return "hip"
+
+    @staticmethod
+    def supports_overloading() -> bool:
+        return True
+
+    @staticmethod
+    def generate(name: str, sol: Mechanism,
+                 opts: CodeGenerationOptions = None) -> str:
+        if opts is None:
+            opts = CodeGenerationOptions()
+        if opts.directive_offload is not None:
+            raise TypeError(
+                "OpenMP/ACC directive based offloading is not supported for "
+                "HIP code generation")
+        if opts.scalar_type not in (None, "double"):
+            raise TypeError("the hip target computes in double precision only")
+        mech = sol
+        L = _Lowering(mech)
+        w = _Writer()
+        wdot_block = 256
+        _emit_prelude(w, L, name, mech.digest(), wdot_block)
+        _emit_thermo(w, L)
+        _emit_wdot(w, L)
+        _emit_reaction_quantities(w, L)
+        _emit_creation_destruction(w, L)
+        w("}  // namespace mech")
+        w("")
+        w('#include "pyro_thermo_kernels.hpp"')
+        return w.text()
+
+    @staticmethod
+    def compile_class(name: str, source: str, mech: Mechanism = None):
+        """Compile generated HIP ``source`` into a shared library and return
+        the device ``Thermochemistry`` class bound to it (the counterpart of
+        ``PythonCodeGenerator.compile_class``, ``codegen/python.py:689-697``)."""
+        from ..runtime import build_mechanism_library
+        from ..thermochem import make_thermochem_class
+        lib = build_mechanism_library(name, source)
+        return make_thermochem_class(name, lib, mech)
+
+    @staticmethod
+    def get_thermochem_class(sol: Mechanism):
+        """Generate, compile and load the class for ``sol``
+        (``codegen/python.py:699-707``)."""
+        from ..runtime import load_mechanism_library
+        from ..thermochem import make_thermochem_class
+        lib = load_mechanism_library(sol)
+        return make_thermochem_class("Thermochemistry", lib, sol)

@@ -1,0 +1,54 @@
+// Common device helpers shared by every generated mechanism translation unit
+// and by the mechanism-independent flamelet library.
+//
+// Target: gfx950 (MI355X), wave64, FP64 arithmetic throughout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pyro {
+
+// Status codes returned across the C ABI.  0 = ok, < 0 = error; positive
+// values are reserved for "number of lanes that did not converge".
+enum Status : int {
+  kOk = 0,
+  kInvalidArgument = -1,
+  kLaunchError = -2,
+  kUnsupported = -3,
+};
+
+constexpr double kGasConstant = 8314.46261815324;  // J/(kmol K)
+constexpr double kOneAtm = 101325.0;               // Pa
+constexpr double kLn10 = 2.302585092994045684;
+
+// One cell per lane: flat 1-D grid over the cell index.  Launches stay far
+// below 2^31 blocks for any cell count that fits in HBM.
+__device__ __forceinline__ int64_t cell_index() {
+  return static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+}
+
+inline int launch_status() {
+  return hipGetLastError() == hipSuccess ? kOk : kLaunchError;
+}
+
+inline unsigned grid_for(int64_t n, int block) {
+  return static_cast<unsigned>((n + block - 1) / block);
+}
+
+// Integer power for small exponents, exact in the same sense as repeated
+// multiplication (used for reaction orders and Arrhenius T^b with integer b).
+template <int P>
+__device__ __forceinline__ double ipow(double x) {
+  if constexpr (P == 0) {
+    return 1.0;
+  } else if constexpr (P < 0) {
+    return 1.0 / ipow<-P>(x);
+  } else if constexpr (P == 1) {
+    return x;
+  } else {
+    return x * ipow<P - 1>(x);
+  }
+}
+
+}  // namespace pyro

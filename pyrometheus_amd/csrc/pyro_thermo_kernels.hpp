@@ -1,0 +1,406 @@
+// Mechanism-generic pointwise kernels and their C ABI.
+//
+// This header is included at the end of every generated mechanism
+// translation unit, after the generator has defined namespace `mech` with:
+//   NS, NR, NF, kW[NS], kInvW[NS]
+//   species_cp_r / species_h_rt / species_s_r / species_g_rt
+//   species_dcp_r / species_dh_rt / species_ds_r
+//   reaction_quantities(...)   (k_f, -ln K_c, r_f, r_r, r_net per reaction)
+//   net_production_rates(...)  (fused omega-dot)
+//   creation_destruction(...)
+//   net_production_rates_jac(...) (omega-dot and its (rho,T,Y) Jacobian)
+// Layout: every per-cell array is SoA; a per-species array is (NS, ld) with
+// cell i of species k at [k*ld + i].  Per-cell scalar inputs take a stride
+// (0 broadcasts one value, 1 reads one value per cell).
+//
+// All entry points are asynchronous on the given stream and return a status
+// (0 ok, <0 error).  Pointers are device pointers owned by the caller.
+#pragma once
+
+#include "pyro_common.hpp"
+
+namespace pyro_kern {
+
+using namespace pyro;
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// species thermo: out[k*ld + i]
+// ---------------------------------------------------------------------------
+enum SpeciesThermo : int {
+  kCpR = 0, kHRT = 1, kSR = 2, kGRT = 3, kDCpR = 4, kDHRT = 5, kDSR = 6,
+};
+
+template <int WHICH>
+__global__ void __launch_bounds__(kBlock)
+k_species_thermo(const double* __restrict__ T, int64_t t_stride,
+                 double* __restrict__ out, int64_t ld, int64_t n) {
+  const int64_t i = cell_index();
+  if (i >= n) return;
+  const double t = T[i * t_stride];
+  double o[mech::NS];
+  if constexpr (WHICH == kCpR) mech::species_cp_r(t, o);
+  if constexpr (WHICH == kHRT) mech::species_h_rt(t, 1.0 / t, o);
+  if constexpr (WHICH == kSR) mech::species_s_r(t, log(t), o);
+  if constexpr (WHICH == kGRT) mech::species_g_rt(t, log(t), 1.0 / t, o);
+  if constexpr (WHICH == kDCpR) mech::species_dcp_r(t, o);
+  if constexpr (WHICH == kDHRT) mech::species_dh_rt(t, 1.0 / t, o);
+  if constexpr (WHICH == kDSR) mech::species_ds_r(t, 1.0 / t, o);
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k) out[k * ld + i] = o[k];
+}
+
+// ---------------------------------------------------------------------------
+// mixture thermo (codegen/python.py:287-311)
+// ---------------------------------------------------------------------------
+enum MixtureThermo : int { kCpMass = 0, kCvMass = 1, kHMass = 2, kEMass = 3 };
+
+// sum_k Y_k * prop_k / W_k
+__device__ __forceinline__ double mass_average(const double* __restrict__ Y,
+                                               int64_t ldY, int64_t i,
+                                               const double (&prop)[mech::NS],
+                                               double shift) {
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k)
+    acc += Y[k * ldY + i] * (prop[k] - shift) * mech::kInvW[k];
+  return acc;
+}
+
+__device__ __forceinline__ void mix_h_cp(double t, const double* __restrict__ Y,
+                                         int64_t ldY, int64_t i, bool energy,
+                                         double& he, double& pv) {
+  // he = e or h (mass), pv = cv or cp (mass), from one pass over species
+  double h[mech::NS], cp[mech::NS];
+  mech::species_h_rt(t, 1.0 / t, h);
+  mech::species_cp_r(t, cp);
+  const double shift = energy ? 1.0 : 0.0;
+  double sh = 0.0, sc = 0.0;
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k) {
+    const double yw = Y[k * ldY + i] * mech::kInvW[k];
+    sh += yw * (h[k] - shift);
+    sc += yw * (cp[k] - shift);
+  }
+  he = kGasConstant * t * sh;
+  pv = kGasConstant * sc;
+}
+
+template <int WHICH>
+__global__ void __launch_bounds__(kBlock)
+k_mixture_thermo(const double* __restrict__ T, int64_t t_stride,
+                 const double* __restrict__ Y, int64_t ldY,
+                 double* __restrict__ out, int64_t n) {
+  const int64_t i = cell_index();
+  if (i >= n) return;
+  const double t = T[i * t_stride];
+  double p[mech::NS];
+  if constexpr (WHICH == kCpMass || WHICH == kCvMass) {
+    mech::species_cp_r(t, p);
+    out[i] = kGasConstant *
+             mass_average(Y, ldY, i, p, WHICH == kCvMass ? 1.0 : 0.0);
+  } else {
+    mech::species_h_rt(t, 1.0 / t, p);
+    out[i] = kGasConstant * t *
+             mass_average(Y, ldY, i, p, WHICH == kEMass ? 1.0 : 0.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// mixture state (codegen/python.py:248-270)
+// ---------------------------------------------------------------------------
+enum MixtureState : int {
+  kMixMolWeight = 0, kSpecificGasConstant = 1, kDensity = 2, kPressure = 3,
+};
+
+__device__ __forceinline__ double inv_w_sum(const double* __restrict__ Y,
+                                            int64_t ldY, int64_t i) {
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k) acc += mech::kInvW[k] * Y[k * ldY + i];
+  return acc;
+}
+
+template <int WHICH>
+__global__ void __launch_bounds__(kBlock)
+k_mixture_state(const double* __restrict__ a, int64_t a_stride,
+                const double* __restrict__ T, int64_t t_stride,
+                const double* __restrict__ Y, int64_t ldY,
+                double* __restrict__ out, int64_t n) {
+  const int64_t i = cell_index();
+  if (i >= n) return;
+  const double s = inv_w_sum(Y, ldY, i);
+  if constexpr (WHICH == kMixMolWeight) out[i] = 1.0 / s;
+  if constexpr (WHICH == kSpecificGasConstant) out[i] = kGasConstant * s;
+  if constexpr (WHICH == kDensity) {
+    const double mmw = 1.0 / s;
+    out[i] = a[i * a_stride] * mmw / (kGasConstant * T[i * t_stride]);
+  }
+  if constexpr (WHICH == kPressure) {
+    const double mmw = 1.0 / s;
+    out[i] = a[i * a_stride] * (kGasConstant * T[i * t_stride]) / mmw;
+  }
+}
+
+// species state: concentrations C_k = invW_k*rho*Y_k, mole fractions
+// X_k = invW_k*Y_k*mmw (codegen/python.py:272-285)
+enum SpeciesState : int { kConcentrations = 0, kMoleFractions = 1 };
+
+template <int WHICH>
+__global__ void __launch_bounds__(kBlock)
+k_species_state(const double* __restrict__ a, int64_t a_stride,
+                const double* __restrict__ Y, int64_t ldY,
+                double* __restrict__ out, int64_t ld, int64_t n) {
+  const int64_t i = cell_index();
+  if (i >= n) return;
+  const double s = a[i * a_stride];
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k) {
+    if constexpr (WHICH == kConcentrations)
+      out[k * ld + i] = mech::kInvW[k] * s * Y[k * ldY + i];
+    else
+      out[k * ld + i] = mech::kInvW[k] * Y[k * ldY + i] * s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kinetics
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load_conc(const double* __restrict__ rho,
+                                          const double* __restrict__ Y,
+                                          int64_t ldY, int64_t i,
+                                          double (&C)[mech::NS]) {
+  const double r = rho[i];
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k) C[k] = mech::kInvW[k] * r * Y[k * ldY + i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_reaction_quantities(const double* __restrict__ T,
+                      const double* __restrict__ C, int64_t ldC,
+                      double* __restrict__ kf, double* __restrict__ nlk,
+                      double* __restrict__ rf, double* __restrict__ rr,
+                      double* __restrict__ rnet, int64_t ldR, int64_t n) {
+  const int64_t i = cell_index();
+  if (i >= n) return;
+  double c[mech::NS];
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k) c[k] = C[k * ldC + i];
+  mech::reaction_quantities(T[i], c, i, ldR, kf, nlk, rf, rr, rnet);
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+k_wdot(const double* __restrict__ rho, const double* __restrict__ T,
+       const double* __restrict__ Y, int64_t ldY, double* __restrict__ W,
+       int64_t ldW, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  double C[mech::NS], w[mech::NS];
+  load_conc(rho, Y, ldY, i, C);
+  mech::net_production_rates(T[i], C, w);
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k) W[k * ldW + i] = w[k];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_creation_destruction(const double* __restrict__ rho,
+                       const double* __restrict__ T,
+                       const double* __restrict__ Y, int64_t ldY,
+                       double* __restrict__ cdot, double* __restrict__ ddot,
+                       int64_t ldW, int64_t n) {
+  const int64_t i = cell_index();
+  if (i >= n) return;
+  double C[mech::NS], cd[mech::NS], dd[mech::NS];
+  load_conc(rho, Y, ldY, i, C);
+  mech::creation_destruction(T[i], C, cd, dd);
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k) {
+    cdot[k * ldW + i] = cd[k];
+    ddot[k * ldW + i] = dd[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// temperature: per-lane Newton on e(T,Y)=e* (do_energy) or h(T,Y)=h*
+// (codegen/python.py:375-396; flamelets/make_pyro.py:255-299).
+// mode 0: step, then stop when |step| < tol (generated get_temperature)
+// mode 1: stop when |step| <= tol without taking it (JAX while_loop form)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock)
+k_temperature(const double* __restrict__ target, int64_t target_stride,
+              const double* __restrict__ t_guess, int64_t guess_stride,
+              const double* __restrict__ Y, int64_t ldY,
+              double* __restrict__ T_out, int64_t n, int do_energy, int mode,
+              double tol, int maxit, int* __restrict__ n_fail,
+              int* __restrict__ iters) {
+  const int64_t i = cell_index();
+  if (i >= n) return;
+  const double e = target[i * target_stride];
+  double t = t_guess[i * guess_stride];
+  const bool energy = do_energy != 0;
+  int it = 0;
+  bool ok = false;
+  for (; it < maxit; ++it) {
+    double he, pv;
+    mix_h_cp(t, Y, ldY, i, energy, he, pv);
+    const double dt = (e - he) / pv;  // = -rhs/deriv with deriv = -pv
+    if (mode == 1) {
+      if (fabs(dt) <= tol) { ok = true; break; }
+      t += dt;
+    } else {
+      t += dt;
+      if (fabs(dt) < tol) { ok = true; ++it; break; }
+    }
+  }
+  T_out[i] = t;
+  if (iters) iters[i] = it;
+  if (!ok && n_fail) atomicAdd(n_fail, 1);
+}
+
+}  // namespace pyro_kern
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+#define PYRO_API extern "C" __attribute__((visibility("default")))
+
+PYRO_API int pyro_mech_info(int64_t* out, int n) {
+  if (!out || n < 4) return pyro::kInvalidArgument;
+  out[0] = mech::NS;
+  out[1] = mech::NR;
+  out[2] = mech::NF;
+  out[3] = mech::kAbiVersion;
+  return pyro::kOk;
+}
+
+PYRO_API int pyro_species_thermo(int which, const double* T, int64_t t_stride,
+                                 double* out, int64_t ld, int64_t n,
+                                 hipStream_t stream) {
+  using namespace pyro_kern;
+  if (n < 0 || ld < n || !T || !out) return pyro::kInvalidArgument;
+  if (n == 0) return pyro::kOk;
+  const dim3 g(pyro::grid_for(n, kBlock)), b(kBlock);
+  switch (which) {
+    case kCpR: k_species_thermo<kCpR><<<g, b, 0, stream>>>(T, t_stride, out, ld, n); break;
+    case kHRT: k_species_thermo<kHRT><<<g, b, 0, stream>>>(T, t_stride, out, ld, n); break;
+    case kSR: k_species_thermo<kSR><<<g, b, 0, stream>>>(T, t_stride, out, ld, n); break;
+    case kGRT: k_species_thermo<kGRT><<<g, b, 0, stream>>>(T, t_stride, out, ld, n); break;
+    case kDCpR: k_species_thermo<kDCpR><<<g, b, 0, stream>>>(T, t_stride, out, ld, n); break;
+    case kDHRT: k_species_thermo<kDHRT><<<g, b, 0, stream>>>(T, t_stride, out, ld, n); break;
+    case kDSR: k_species_thermo<kDSR><<<g, b, 0, stream>>>(T, t_stride, out, ld, n); break;
+    default: return pyro::kInvalidArgument;
+  }
+  return pyro::launch_status();
+}
+
+PYRO_API int pyro_mixture_thermo(int which, const double* T, int64_t t_stride,
+                                 const double* Y, int64_t ldY, double* out,
+                                 int64_t n, hipStream_t stream) {
+  using namespace pyro_kern;
+  if (n < 0 || ldY < n || !T || !Y || !out) return pyro::kInvalidArgument;
+  if (n == 0) return pyro::kOk;
+  const dim3 g(pyro::grid_for(n, kBlock)), b(kBlock);
+  switch (which) {
+    case kCpMass: k_mixture_thermo<kCpMass><<<g, b, 0, stream>>>(T, t_stride, Y, ldY, out, n); break;
+    case kCvMass: k_mixture_thermo<kCvMass><<<g, b, 0, stream>>>(T, t_stride, Y, ldY, out, n); break;
+    case kHMass: k_mixture_thermo<kHMass><<<g, b, 0, stream>>>(T, t_stride, Y, ldY, out, n); break;
+    case kEMass: k_mixture_thermo<kEMass><<<g, b, 0, stream>>>(T, t_stride, Y, ldY, out, n); break;
+    default: return pyro::kInvalidArgument;
+  }
+  return pyro::launch_status();
+}
+
+PYRO_API int pyro_mixture_state(int which, const double* a, int64_t a_stride,
+                                const double* T, int64_t t_stride,
+                                const double* Y, int64_t ldY, double* out,
+                                int64_t n, hipStream_t stream) {
+  using namespace pyro_kern;
+  if (n < 0 || ldY < n || !Y || !out) return pyro::kInvalidArgument;
+  if ((which == kDensity || which == kPressure) && (!a || !T))
+    return pyro::kInvalidArgument;
+  if (n == 0) return pyro::kOk;
+  const dim3 g(pyro::grid_for(n, kBlock)), b(kBlock);
+  switch (which) {
+    case kMixMolWeight: k_mixture_state<kMixMolWeight><<<g, b, 0, stream>>>(a, a_stride, T, t_stride, Y, ldY, out, n); break;
+    case kSpecificGasConstant: k_mixture_state<kSpecificGasConstant><<<g, b, 0, stream>>>(a, a_stride, T, t_stride, Y, ldY, out, n); break;
+    case kDensity: k_mixture_state<kDensity><<<g, b, 0, stream>>>(a, a_stride, T, t_stride, Y, ldY, out, n); break;
+    case kPressure: k_mixture_state<kPressure><<<g, b, 0, stream>>>(a, a_stride, T, t_stride, Y, ldY, out, n); break;
+    default: return pyro::kInvalidArgument;
+  }
+  return pyro::launch_status();
+}
+
+PYRO_API int pyro_species_state(int which, const double* a, int64_t a_stride,
+                                const double* Y, int64_t ldY, double* out,
+                                int64_t ld, int64_t n, hipStream_t stream) {
+  using namespace pyro_kern;
+  if (n < 0 || ldY < n || ld < n || !a || !Y || !out)
+    return pyro::kInvalidArgument;
+  if (n == 0) return pyro::kOk;
+  const dim3 g(pyro::grid_for(n, kBlock)), b(kBlock);
+  switch (which) {
+    case kConcentrations: k_species_state<kConcentrations><<<g, b, 0, stream>>>(a, a_stride, Y, ldY, out, ld, n); break;
+    case kMoleFractions: k_species_state<kMoleFractions><<<g, b, 0, stream>>>(a, a_stride, Y, ldY, out, ld, n); break;
+    default: return pyro::kInvalidArgument;
+  }
+  return pyro::launch_status();
+}
+
+PYRO_API int pyro_reaction_quantities(const double* T, const double* C,
+                                      int64_t ldC, double* kf, double* nlk,
+                                      double* rf, double* rr, double* rnet,
+                                      int64_t ldR, int64_t n,
+                                      hipStream_t stream) {
+  using namespace pyro_kern;
+  if (n < 0 || ldC < n || ldR < n || !T || !C) return pyro::kInvalidArgument;
+  if (n == 0) return pyro::kOk;
+  k_reaction_quantities<<<pyro::grid_for(n, kBlock), kBlock, 0, stream>>>(
+      T, C, ldC, kf, nlk, rf, rr, rnet, ldR, n);
+  return pyro::launch_status();
+}
+
+PYRO_API int pyro_wdot(const double* rho, const double* T, const double* Y,
+                       int64_t ldY, double* wdot, int64_t ldW, int64_t n,
+                       hipStream_t stream) {
+  using namespace pyro_kern;
+  if (n < 0 || ldY < n || ldW < n || !rho || !T || !Y || !wdot)
+    return pyro::kInvalidArgument;
+  if (n == 0) return pyro::kOk;
+  constexpr int B = mech::kWdotBlock;
+  k_wdot<B><<<pyro::grid_for(n, B), B, 0, stream>>>(rho, T, Y, ldY, wdot, ldW,
+                                                    n);
+  return pyro::launch_status();
+}
+
+PYRO_API int pyro_creation_destruction(const double* rho, const double* T,
+                                       const double* Y, int64_t ldY,
+                                       double* cdot, double* ddot,
+                                       int64_t ldW, int64_t n,
+                                       hipStream_t stream) {
+  using namespace pyro_kern;
+  if (n < 0 || ldY < n || ldW < n || !rho || !T || !Y || !cdot || !ddot)
+    return pyro::kInvalidArgument;
+  if (n == 0) return pyro::kOk;
+  k_creation_destruction<<<pyro::grid_for(n, kBlock), kBlock, 0, stream>>>(
+      rho, T, Y, ldY, cdot, ddot, ldW, n);
+  return pyro::launch_status();
+}
+
+PYRO_API int pyro_temperature(const double* target, int64_t target_stride,
+                              const double* t_guess, int64_t guess_stride,
+                              const double* Y, int64_t ldY, double* T_out,
+                              int64_t n, int do_energy, int mode, double tol,
+                              int maxit, int* n_fail, int* iters,
+                              hipStream_t stream) {
+  using namespace pyro_kern;
+  if (n < 0 || ldY < n || !target || !t_guess || !Y || !T_out || maxit < 0 ||
+      !(tol >= 0.0))
+    return pyro::kInvalidArgument;
+  if (n == 0) return pyro::kOk;
+  k_temperature<<<pyro::grid_for(n, kBlock), kBlock, 0, stream>>>(
+      target, target_stride, t_guess, guess_stride, Y, ldY, T_out, n,
+      do_energy, mode, tol, maxit, n_fail, iters);
+  return pyro::launch_status();
+}

@@ -1,0 +1,59 @@
+"""The C-ABI libraries load on a CPU-only host and export every symbol the
+public headers declare (no compute calls: those need a GPU)."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from pyrometheus_amd import bundled, get_code_generators
+from pyrometheus_amd.codegen import CodeGenerationOptions
+from pyrometheus_amd.codegen.hip import HipCodeGenerator
+from pyrometheus_amd import runtime
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"^int\s+(pyro_\w+)\s*\(", text, re.M)))
+
+
+def test_registry_and_options():
+    gens = get_code_generators()
+    assert list(gens) == ["hip"]
+    assert gens["hip"].supports_overloading()
+    with pytest.raises(TypeError):
+        HipCodeGenerator.generate("x", bundled("sandiego"),
+                                  CodeGenerationOptions(directive_offload="acc"))
+
+
+def test_generated_source_shape():
+    src = HipCodeGenerator.generate("sandiego", bundled("sandiego"))
+    assert "constexpr int NS = 9;" in src and "constexpr int NR = 24;" in src
+    assert src.count("// R") >= 24
+    assert '#include "pyro_thermo_kernels.hpp"' in src
+
+
+def test_mechanism_library_exports():
+    mech = bundled("sandiego")
+    so = runtime.build_mechanism_library(mech)
+    lib = ctypes.CDLL(so, mode=os.RTLD_LOCAL)
+    names = _declared("pyro_hip.h")
+    assert names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the loader binds every declared symbol with a signature
+    assert set(names) <= set(runtime.MECH_SIGNATURES)
+
+
+def test_core_library_exports():
+    if not os.path.exists(os.path.join(ROOT, "include", "pyro_core.h")):
+        pytest.skip("no core header")
+    so = runtime.build_core_library()
+    lib = ctypes.CDLL(so, mode=os.RTLD_LOCAL)
+    names = _declared("pyro_core.h")
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(runtime.CORE_SIGNATURES)
