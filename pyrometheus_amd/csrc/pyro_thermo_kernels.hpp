@@ -279,8 +279,8 @@ PYRO_API int pyro_species_thermo(int which, const double* T, int64_t t_stride,
                                  double* out, int64_t ld, int64_t n,
                                  hipStream_t stream) {
   using namespace pyro_kern;
-  if (n < 0 || ld < n || !T || !out) return pyro::kInvalidArgument;
   if (n == 0) return pyro::kOk;
+  if (n < 0 || ld < n || !T || !out) return pyro::kInvalidArgument;
   const dim3 g(pyro::grid_for(n, kBlock)), b(kBlock);
   switch (which) {
     case kCpR: k_species_thermo<kCpR><<<g, b, 0, stream>>>(T, t_stride, out, ld, n); break;
@@ -299,8 +299,8 @@ PYRO_API int pyro_mixture_thermo(int which, const double* T, int64_t t_stride,
                                  const double* Y, int64_t ldY, double* out,
                                  int64_t n, hipStream_t stream) {
   using namespace pyro_kern;
-  if (n < 0 || ldY < n || !T || !Y || !out) return pyro::kInvalidArgument;
   if (n == 0) return pyro::kOk;
+  if (n < 0 || ldY < n || !T || !Y || !out) return pyro::kInvalidArgument;
   const dim3 g(pyro::grid_for(n, kBlock)), b(kBlock);
   switch (which) {
     case kCpMass: k_mixture_thermo<kCpMass><<<g, b, 0, stream>>>(T, t_stride, Y, ldY, out, n); break;
@@ -317,10 +317,10 @@ PYRO_API int pyro_mixture_state(int which, const double* a, int64_t a_stride,
                                 const double* Y, int64_t ldY, double* out,
                                 int64_t n, hipStream_t stream) {
   using namespace pyro_kern;
+  if (n == 0) return pyro::kOk;
   if (n < 0 || ldY < n || !Y || !out) return pyro::kInvalidArgument;
   if ((which == kDensity || which == kPressure) && (!a || !T))
     return pyro::kInvalidArgument;
-  if (n == 0) return pyro::kOk;
   const dim3 g(pyro::grid_for(n, kBlock)), b(kBlock);
   switch (which) {
     case kMixMolWeight: k_mixture_state<kMixMolWeight><<<g, b, 0, stream>>>(a, a_stride, T, t_stride, Y, ldY, out, n); break;
@@ -336,9 +336,9 @@ PYRO_API int pyro_species_state(int which, const double* a, int64_t a_stride,
                                 const double* Y, int64_t ldY, double* out,
                                 int64_t ld, int64_t n, hipStream_t stream) {
   using namespace pyro_kern;
+  if (n == 0) return pyro::kOk;
   if (n < 0 || ldY < n || ld < n || !a || !Y || !out)
     return pyro::kInvalidArgument;
-  if (n == 0) return pyro::kOk;
   const dim3 g(pyro::grid_for(n, kBlock)), b(kBlock);
   switch (which) {
     case kConcentrations: k_species_state<kConcentrations><<<g, b, 0, stream>>>(a, a_stride, Y, ldY, out, ld, n); break;
@@ -354,8 +354,8 @@ PYRO_API int pyro_reaction_quantities(const double* T, const double* C,
                                       int64_t ldR, int64_t n,
                                       hipStream_t stream) {
   using namespace pyro_kern;
-  if (n < 0 || ldC < n || ldR < n || !T || !C) return pyro::kInvalidArgument;
   if (n == 0) return pyro::kOk;
+  if (n < 0 || ldC < n || ldR < n || !T || !C) return pyro::kInvalidArgument;
   k_reaction_quantities<<<pyro::grid_for(n, kBlock), kBlock, 0, stream>>>(
       T, C, ldC, kf, nlk, rf, rr, rnet, ldR, n);
   return pyro::launch_status();
@@ -365,9 +365,9 @@ PYRO_API int pyro_wdot(const double* rho, const double* T, const double* Y,
                        int64_t ldY, double* wdot, int64_t ldW, int64_t n,
                        hipStream_t stream) {
   using namespace pyro_kern;
+  if (n == 0) return pyro::kOk;
   if (n < 0 || ldY < n || ldW < n || !rho || !T || !Y || !wdot)
     return pyro::kInvalidArgument;
-  if (n == 0) return pyro::kOk;
   constexpr int B = mech::kWdotBlock;
   k_wdot<B><<<pyro::grid_for(n, B), B, 0, stream>>>(rho, T, Y, ldY, wdot, ldW,
                                                     n);
@@ -380,9 +380,9 @@ PYRO_API int pyro_creation_destruction(const double* rho, const double* T,
                                        int64_t ldW, int64_t n,
                                        hipStream_t stream) {
   using namespace pyro_kern;
+  if (n == 0) return pyro::kOk;
   if (n < 0 || ldY < n || ldW < n || !rho || !T || !Y || !cdot || !ddot)
     return pyro::kInvalidArgument;
-  if (n == 0) return pyro::kOk;
   k_creation_destruction<<<pyro::grid_for(n, kBlock), kBlock, 0, stream>>>(
       rho, T, Y, ldY, cdot, ddot, ldW, n);
   return pyro::launch_status();
@@ -395,10 +395,10 @@ PYRO_API int pyro_temperature(const double* target, int64_t target_stride,
                               int maxit, int* n_fail, int* iters,
                               hipStream_t stream) {
   using namespace pyro_kern;
+  if (n == 0) return pyro::kOk;
   if (n < 0 || ldY < n || !target || !t_guess || !Y || !T_out || maxit < 0 ||
       !(tol >= 0.0))
     return pyro::kInvalidArgument;
-  if (n == 0) return pyro::kOk;
   k_temperature<<<pyro::grid_for(n, kBlock), kBlock, 0, stream>>>(
       target, target_stride, t_guess, guess_stride, Y, ldY, T_out, n,
       do_energy, mode, tol, maxit, n_fail, iters);

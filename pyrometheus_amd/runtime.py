@@ -61,8 +61,29 @@ def compile_hip(src_path: str, out_path: str, extra=()) -> None:
     os.replace(out_path + ".tmp", out_path)
 
 
+_SRC_DIGEST = None
+
+
+def source_digest() -> str:
+    """Hash of everything a generated library is compiled from besides the
+    mechanism: the hand-written headers, the generator and the flags."""
+    global _SRC_DIGEST
+    if _SRC_DIGEST is None:
+        import hashlib
+        h = hashlib.sha256(" ".join(HIPCC_FLAGS).encode())
+        files = sorted(f for f in os.listdir(CSRC_DIR) if f.endswith(".hpp"))
+        for f in files:
+            with open(os.path.join(CSRC_DIR, f), "rb") as fh:
+                h.update(fh.read())
+        with open(os.path.join(PKG_DIR, "codegen", "hip.py"), "rb") as fh:
+            h.update(fh.read())
+        _SRC_DIGEST = h.hexdigest()[:8]
+    return _SRC_DIGEST
+
+
 def mechanism_lib_path(mech: Mechanism) -> str:
-    return os.path.join(GEN_DIR, f"libpyro_{mech.name}_{mech.digest()}.so")
+    return os.path.join(
+        GEN_DIR, f"libpyro_{mech.name}_{mech.digest()}_{source_digest()}.so")
 
 
 def generate_source(mech: Mechanism) -> str:
