@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES "
+                         "(3 launches); calibrates HBM byte counters")
     ap.add_argument("--traffic-json",
                     default=os.path.join(ROOT, "profiles", "wdot_gri30_traffic.json"))
     return ap.parse_args()
@@ -106,6 +109,15 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a ROCm device"
     device = torch.device("cuda:0")
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe // 8, dtype=torch.float64,
+                         device=device)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 8}), flush=True)
+        return
     mech = bundled(args.mech)
     gas = get_thermochem_class(mech)(device=device)
     n = args.cells

@@ -48,6 +48,9 @@ from ..mechanism import GAS_CONSTANT, ONE_ATM, Mechanism, Reaction
 from . import CodeGenerationOptions, CodeGenerator
 
 ABI_VERSION = 1
+#: mechanisms up to this many species also get the dense-Jacobian kernels
+#: (the NS x NS Jacobian lives in registers) and the flamelet kernels
+DENSE_JAC_MAX_SPECIES = 16
 CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                     "csrc")
 
@@ -89,15 +92,81 @@ class _Writer:
         return "\n".join(self.lines) + "\n"
 
 
+#: Code-shape choices of the fused kinetics kernels.
+#:  pin   -- pin each species accumulator after every reaction (an empty asm
+#:           with a read-write operand), so the compiler cannot sink the
+#:           accumulation adds to the final stores and keep every rate live;
+#:  kc    -- "species": K_c^-1 as a product of per-species exp(g'_k) factors
+#:           (one exp per species instead of one per reaction); "exp": one
+#:           exp of -ln K_c per reversible reaction;
+DEFAULT_TUNING = {"pin": True, "pin_every": 4, "kc": "species", "ctable": True}
+
+
+def element_shift(mech: Mechanism):
+    """Integer element potentials (mu_e, nu_e) that centre the shifted Gibbs
+    functions g'_k = g_k - sum_e a_ek (mu_e / T + nu_e) near zero over
+    300..3500 K (least squares, rounded to integers so every per-species
+    shift is exact).  For an element-balanced reaction sum nu g' = sum nu g,
+    so K_c is unchanged; exp(g'_k) stays far from overflow even at 300 K."""
+    import numpy as np
+    Ts = np.linspace(300.0, 3500.0, 33)
+    rows, rhs = [], []
+    A = mech.element_matrix().T  # (ns, ne)
+    for s_i, sp in enumerate(mech.species):
+        for t in Ts:
+            c = sp.high if t > sp.t_mid else sp.low
+            g = (c[0] * (1 - math.log(t)) - c[1] / 2 * t - c[2] / 6 * t ** 2
+                 - c[3] / 12 * t ** 3 - c[4] / 20 * t ** 4 + c[5] / t - c[6])
+            rows.append(np.concatenate([A[s_i] / t, A[s_i]]))
+            rhs.append(g)
+    sol = np.linalg.lstsq(np.array(rows), np.array(rhs), rcond=None)[0]
+    ne = mech.n_elements
+    mu = np.round(sol[:ne])
+    nu = np.round(sol[ne:])
+    return mu, nu
+
+
 class _Lowering:
     """Lowers one mechanism; methods return C++ expressions or emit blocks."""
 
-    def __init__(self, mech: Mechanism):
+    def __init__(self, mech: Mechanism, tuning: Optional[dict] = None):
         self.m = mech
         self.ns = mech.n_species
         self.idx = {s.name: i for i, s in enumerate(mech.species)}
         self.tmids = sorted({s.t_mid for s in mech.species})
         self.tmid_flag = {t: f"hi{j}" for j, t in enumerate(self.tmids)}
+        self.tuning = dict(DEFAULT_TUNING)
+        self.tuning.update(tuning or {})
+        mu, nu = element_shift(mech)
+        A = mech.element_matrix().T
+        self.shift_m = [float(A[k] @ mu) for k in range(self.ns)]
+        self.shift_n = [float(A[k] @ nu) for k in range(self.ns)]
+        E = mech.element_matrix()
+
+        def balanced_integer(r):
+            if r.orders:
+                return False
+            nus = [v for _, v in r.reactants] + [v for _, v in r.products]
+            if not all(float(v).is_integer() and 0 < v <= 4 for v in nus):
+                return False
+            lhs = sum(E[:, self.idx[sp]] * v for sp, v in r.reactants)
+            rhs = sum(E[:, self.idx[sp]] * v for sp, v in r.products)
+            return bool((abs(lhs - rhs) < 1e-12).all())
+        self.species_kc = (self.tuning["kc"] == "species" and all(
+            balanced_integer(r) for r in mech.reactions if r.reversible))
+        self.tab = None  # list of tabulated constants while emitting a table
+
+    _INLINE = {0.0, 0.5, -0.5, 1.0, -1.0, 2.0, -2.0, 4.0, -4.0}
+
+    def c(self, x: float) -> str:
+        """A constant operand: a literal, or an entry of the current constant
+        table (read through scalar loads) when one is being emitted."""
+        x = float(x)
+        if self.tab is None or x in self._INLINE or (
+                x.is_integer() and -16 <= x <= 64):
+            return lit(x)
+        self.tab.append(x)
+        return f"K[{len(self.tab) - 1}]"
 
     # --- thermo ---------------------------------------------------------
     def _flags(self, w: _Writer, T: str = "T"):
@@ -170,20 +239,91 @@ class _Lowering:
         cs = [self._sel(sp, hi[j], lo[j]) for j in (1, 2, 3, 4)]
         return f"fma({self._sel(sp, hi[0], lo[0])}, invT, {self._horner(cs, 'T')})"
 
+    # --- shifted Gibbs exponentials --------------------------------------
+    def emit_gibbs_exp(self, w: _Writer):
+        """E[k] = exp(g'_k): both NASA ranges in the basis
+        (1 - ln T, T, T^2, T^3, T^4, 1/T, 1), one select per species."""
+        w("const double omlnT = 1.0 - lnT;")
+        w("const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;")
+        self._flags(w)
+        w("double E[NS];")
+        for k, sp in enumerate(self.m.species):
+            def coeffs(c):
+                return [c[0], -c[1] / 2, -c[2] / 6, -c[3] / 12, -c[4] / 20,
+                        c[5] - self.shift_m[k], -(c[6] + self.shift_n[k])]
+            basis = ["omlnT", "T", "T2", "T3", "T4", "invT"]
+
+            def poly(cs):
+                expr = self.c(cs[6])
+                for cc, b in zip(reversed(cs[:6]), reversed(basis)):
+                    expr = f"fma({self.c(cc)}, {b}, {expr})"
+                return expr
+            hi, lo = coeffs(sp.high), coeffs(sp.low)
+            if hi == lo:
+                w(f"E[{k}] = exp({poly(hi)});  // {sp.name}")
+            else:
+                # both ranges evaluated and pinned so the choice stays a
+                # per-lane select rather than a divergent branch
+                w(f"{{ double gh = {poly(hi)}, gl = {poly(lo)};")
+                w('  asm volatile("" : "+v"(gh), "+v"(gl));')
+                w(f"  E[{k}] = exp({self.tmid_flag[sp.t_mid]} ? gh : gl); }}"
+                  f"  // {sp.name}")
+        w("const double rtp = kGasConstant * T / kOneAtm;")
+        w("const double prt = kOneAtm / (kGasConstant * T);")
+
+    def kinv_expr(self, r: Reaction) -> str:
+        """exp(-ln K_c) from the E[] factors (balanced, integer nu)."""
+        def side(d, extra):
+            ts = []
+            for sp, nu in d:
+                ts += [f"E[{self.idx[sp]}]"] * int(nu)
+            ts += extra
+            return " * ".join(ts) if ts else "1.0"
+        dnu = int(sum(nu for _, nu in r.products)
+                  - sum(nu for _, nu in r.reactants))
+        num_extra = ["rtp"] * dnu if dnu > 0 else []
+        den_extra = ["rtp"] * (-dnu) if dnu < 0 else []
+        return (f"({side(r.products, num_extra)}) * "
+                f"pyro::recip({side(r.reactants, den_extra)})")
+
+    def pin(self, w: _Writer, names: List[str]):
+        if self.tuning["pin"] and names:
+            ops = ", ".join(f'"+v"({n})' for n in names)
+            w(f'asm volatile("" : {ops});')
+
     # --- kinetics -------------------------------------------------------
-    @staticmethod
-    def arrhenius(rate) -> str:
+    def arrhenius(self, rate) -> str:
         a, b, ta = rate.A, rate.b, rate.t_a
+        c = self.c
         if ta == 0:
             if b == 0:
-                return lit(a)
+                return c(a)
             if _is_small_int(b):
-                return f"({lit(a)} * pyro::ipow<{int(b)}>(T))"
-            return f"({lit(a)} * exp({lit(b)} * lnT))"
+                if b < 0:
+                    return f"({c(a)} * pyro::ipow<{int(-b)}>(invT))"
+                return f"({c(a)} * pyro::ipow<{int(b)}>(T))"
+            return f"({c(a)} * exp({c(b)} * lnT))"
         lna = math.log(a)
         if b == 0:
-            return f"exp(fma({lit(-ta)}, invT, {lit(lna)}))"
-        return f"exp(fma({lit(b)}, lnT, fma({lit(-ta)}, invT, {lit(lna)})))"
+            return f"exp(fma({c(-ta)}, invT, {c(lna)}))"
+        return f"exp(fma({c(b)}, lnT, fma({c(-ta)}, invT, {c(lna)})))"
+
+    def rate_ratio(self, lo, hi) -> str:
+        """k_lo / k_hi as one expression: a constant times an integer power
+        of T when both are pure power laws, else one exp of the difference
+        of the Arrhenius exponents."""
+        db = lo.b - hi.b
+        if lo.t_a == 0 and hi.t_a == 0 and _is_small_int(db):
+            c = self.c(lo.A / hi.A)
+            if db == 0:
+                return c
+            if db < 0:
+                return f"({c} * pyro::ipow<{int(-db)}>(invT))"
+            return f"({c} * pyro::ipow<{int(db)}>(T))"
+        dlna = math.log(lo.A) - math.log(hi.A)
+        dta = lo.t_a - hi.t_a
+        return (f"exp(fma({self.c(db)}, lnT, fma({self.c(-dta)}, invT, "
+                f"{self.c(dlna)})))")
 
     def third_body(self, r: Reaction) -> str:
         d = r.default_efficiency
@@ -192,9 +332,9 @@ class _Lowering:
             coef = eff - d
             if coef != 0.0:
                 terms.append((coef, self.idx[sp]))
-        expr = "0.0" if d == 0 else ("Ctot" if d == 1.0 else f"({lit(d)} * Ctot)")
+        expr = "0.0" if d == 0 else ("Ctot" if d == 1.0 else f"({self.c(d)} * Ctot)")
         for coef, k in terms:
-            expr = f"fma({lit(coef)}, C[{k}], {expr})"
+            expr = f"fma({self.c(coef)}, C[{k}], {expr})"
         return expr
 
     def uses_ctot(self) -> bool:
@@ -206,29 +346,118 @@ class _Lowering:
         falloff and three-body factors (codegen/python.py:398-473)."""
         if r.kind == "falloff":
             w(f"const double khi = {self.arrhenius(r.high_rate)};")
-            w(f"const double klo = {self.arrhenius(r.low_rate)};")
-            w(f"const double pr = ({self.third_body(r)}) * klo / khi;")
+            w(f"const double pr = ({self.third_body(r)}) * "
+              f"{self.rate_ratio(r.low_rate, r.high_rate)};")
             if r.falloff_type == "Troe":
                 tp = r.troe
-                t1 = f"({lit(1.0 - tp[0])} * exp(T * {lit(-1.0 / tp[1])}))"
-                t2 = f"({lit(tp[0])} * exp(T * {lit(-1.0 / tp[2])}))"
+                c = self.c
+                t1 = f"({c(1.0 - tp[0])} * exp(T * {c(-1.0 / tp[1])}))"
+                t2 = f"({c(tp[0])} * exp(T * {c(-1.0 / tp[2])}))"
                 s = f"{t1} + {t2}"
                 if len(tp) == 4:
-                    s += f" + exp({lit(-tp[3])} * invT)"
+                    s += f" + exp({c(-tp[3])} * invT)"
                 w(f"const double fc = log10({s});")
                 w("const double lpr = log10(pr);")
                 w("const double cc = fma(-0.67, fc, -0.4);")
                 w("const double nn = fma(-1.27, fc, 0.75);")
-                w("const double f1 = pr > 0.0 ? (lpr + cc) / fma(-0.14, lpr + cc, nn)"
-                  f" : {lit(-1 / 0.14)};")
-                w("const double fF = exp10(fc / fma(f1, f1, 1.0));")
-                w(f"const double {out} = khi * (fF * pr / (1.0 + pr));")
+                w("const double f1 = pr > 0.0 ? (lpr + cc) * pyro::recip("
+                  f"fma(-0.14, lpr + cc, nn)) : {lit(-1 / 0.14)};")
+                w("const double fF = exp10(fc * pyro::recip(fma(f1, f1, 1.0)));")
+                w(f"const double {out} = khi * (fF * pr * pyro::recip(1.0 + pr));")
             else:
-                w(f"const double {out} = khi * (pr / (1.0 + pr));")
+                w(f"const double {out} = khi * (pr * pyro::recip(1.0 + pr));")
         elif r.kind == "three-body":
             w(f"const double {out} = {self.arrhenius(r.rate)} * ({self.third_body(r)});")
         else:
             w(f"const double {out} = {self.arrhenius(r.rate)};")
+
+    # --- derivatives (forward mode, hand-written per reaction kind) ---------
+    def dlnk(self, rate) -> str:
+        """d ln k / dT = (b + Ta/T) / T."""
+        b, ta = rate.b, rate.t_a
+        if ta == 0:
+            return "0.0" if b == 0 else f"({self.c(b)} * invT)"
+        return f"(invT * fma({self.c(ta)}, invT, {self.c(b)}))"
+
+    def kf_block_deriv(self, w: _Writer, r: Reaction):
+        """Emit kf, dkfT = dkf/dT and (three-body, falloff) M and
+        dkfM = dkf/d[M] (chem_expr.py:310-421 differentiated by hand)."""
+        c = self.c
+        if r.kind == "falloff":
+            ln10 = math.log(10.0)
+            w(f"const double khi = {self.arrhenius(r.high_rate)};")
+            w(f"const double ahi = {self.dlnk(r.high_rate)};")
+            w(f"const double alo = {self.dlnk(r.low_rate)};")
+            w(f"const double ratio = {self.rate_ratio(r.low_rate, r.high_rate)};")
+            w(f"const double M = {self.third_body(r)};")
+            w("const double pr = M * ratio;")
+            w("const double rpr1 = pyro::recip(1.0 + pr);")
+            if r.falloff_type == "Troe":
+                tp = r.troe
+                w(f"const double e1 = exp(T * {c(-1.0 / tp[1])});")
+                w(f"const double e2 = exp(T * {c(-1.0 / tp[2])});")
+                S = f"fma({c(1.0 - tp[0])}, e1, {c(tp[0])} * e2)"
+                dS = (f"fma({c((1.0 - tp[0]) * (-1.0 / tp[1]))}, e1, "
+                      f"{c(tp[0] * (-1.0 / tp[2]))} * e2)")
+                if len(tp) == 4:
+                    w(f"const double e3 = exp({c(-tp[3])} * invT);")
+                    S = f"({S} + e3)"
+                    dS = f"fma({c(tp[3])} * invT * invT, e3, {dS})"
+                w(f"const double sS = {S};")
+                w(f"const double fc = log10(sS);")
+                w(f"const double dfc = {dS} * pyro::recip(sS) * {c(1 / ln10)};")
+                w("const double lpr = log10(pr);")
+                w("const double cc = fma(-0.67, fc, -0.4);")
+                w("const double nn = fma(-1.27, fc, 0.75);")
+                w("const double u = lpr + cc;")
+                w("const double rden = pyro::recip(fma(-0.14, u, nn));")
+                w("const bool pos = pr > 0.0;")
+                w(f"const double f1 = pos ? u * rden : {lit(-1 / 0.14)};")
+                w("const double ropf = pyro::recip(fma(f1, f1, 1.0));")
+                w("const double fF = exp10(fc * ropf);")
+                w("const double dlF_df1 = -2.0 * fc * f1 * ropf * ropf;")
+                w("const double df1_du = pos ? nn * rden * rden : 0.0;")
+                w("const double df1_dn = pos ? -u * rden * rden : 0.0;")
+                w("const double dlF_dL = dlF_df1 * df1_du;")
+                w("const double dlF_dfc = fma(dlF_df1, fma(-0.67, df1_du, "
+                  "-1.27 * df1_dn), ropf);")
+                w(f"const double dlF_dT = fma(dlF_dL * {c(1 / ln10)}, alo - ahi, "
+                  "dlF_dfc * dfc);")
+                w("const double kf = khi * (fF * pr * rpr1);")
+                w("const double dkfM = khi * fF * ratio * fma(dlF_dL, rpr1, "
+                  "rpr1 * rpr1);")
+                w(f"const double dkfT = kf * (ahi + fma({c(ln10)}, dlF_dT, "
+                  "(alo - ahi) * rpr1));")
+            else:
+                w("const double kf = khi * (pr * rpr1);")
+                w("const double dkfM = khi * ratio * rpr1 * rpr1;")
+                w("const double dkfT = kf * fma(alo - ahi, rpr1, ahi);")
+        elif r.kind == "three-body":
+            w(f"const double k0 = {self.arrhenius(r.rate)};")
+            w(f"const double M = {self.third_body(r)};")
+            w("const double kf = k0 * M;")
+            w("const double dkfM = k0;")
+            w(f"const double dkfT = kf * {self.dlnk(r.rate)};")
+        else:
+            w(f"const double kf = {self.arrhenius(r.rate)};")
+            w(f"const double dkfT = kf * {self.dlnk(r.rate)};")
+
+    def prod_partials(self, pairs) -> List[Tuple[int, str]]:
+        """(k, d prod / d C_k) for prod = prod_j C_j^nu_j, written without
+        division so it stays finite when a concentration is zero."""
+        out = []
+        for i, (k, nu) in enumerate(pairs):
+            others = [self._pow_term(kk, nn) for j, (kk, nn) in enumerate(pairs)
+                      if j != i]
+            if nu == 1.0:
+                d = []
+            elif _is_small_int(nu) and nu > 0:
+                d = [lit(nu)] + [f"C[{k}]"] * (int(nu) - 1)
+            else:
+                d = [lit(nu), f"pow(C[{k}], {lit(nu - 1.0)})"]
+            terms = d + others
+            out.append((k, " * ".join(terms) if terms else "1.0"))
+        return out
 
     def _pow_term(self, k: int, nu: float) -> str:
         if nu == 1.0:
@@ -338,12 +567,27 @@ def _kin_prologue(w: _Writer, L: _Lowering, need_g: bool):
 def _emit_wdot(w: _Writer, L: _Lowering):
     m = L.m
     w("// fused net production rates (codegen/python.py:475-495)")
+    table_at = len(w.lines)
     w("__device__ __forceinline__ void net_production_rates(const double T, "
       "const double (&C)[NS], double (&w)[NS]) {")
     w.ind += 1
-    _kin_prologue(w, L, need_g=any(r.reversible for r in m.reactions))
+    if L.tuning.get("ctable", True):
+        L.tab = []
+        # constants are read from a table in constant memory through a
+        # pointer the optimiser cannot see through: uniform scalar loads
+        # replace per-use literal materialisation
+        w("pyro::cptr K = (pyro::cptr)kTabWdot;")
+        w('asm volatile("" : "+s"(K));')
+    rev = any(r.reversible for r in m.reactions)
+    if L.species_kc:
+        _kin_prologue(w, L, need_g=False)
+        if rev:
+            L.emit_gibbs_exp(w)
+    else:
+        _kin_prologue(w, L, need_g=rev)
     w("#pragma unroll")
     w("for (int k = 0; k < NS; ++k) w[k] = 0.0;")
+    pending = set()
     for i, r in enumerate(m.reactions):
         w(f"{{  // R{i}: {r.equation} [{r.reaction_type}]")
         w.ind += 1
@@ -351,18 +595,120 @@ def _emit_wdot(w: _Writer, L: _Lowering):
         rf = L.prod_expr(L.reactant_pairs(r))
         if r.reversible:
             rr = L.prod_expr([(L.idx[sp], nu) for sp, nu in r.products])
-            w(f"const double q = kf * ({rf} - exp({L.nlk_expr(r)}) * {rr});")
+            kinv = L.kinv_expr(r) if L.species_kc else f"exp({L.nlk_expr(r)})"
+            w(f"const double q = kf * ({rf} - {kinv} * {rr});")
         else:
             w(f"const double q = kf * ({rf});")
-        for k, nu in L.species_updates(r, "q"):
+        upd = L.species_updates(r, "q")
+        for k, nu in upd:
             if nu == 1.0:
                 w(f"w[{k}] += q;")
             elif nu == -1.0:
                 w(f"w[{k}] -= q;")
             else:
                 w(f"w[{k}] = fma({lit(nu)}, q, w[{k}]);")
+        pending.update(k for k, _ in upd)
+        if (i + 1) % L.tuning.get("pin_every", 1) == 0 or i + 1 == len(m.reactions):
+            L.pin(w, [f"w[{k}]" for k in sorted(pending)])
+            pending.clear()
         w.ind -= 1
         w("}")
+    w.ind -= 1
+    w("}")
+    w("")
+    if L.tab is not None:
+        tab = L.tab or [0.0]
+        decl = ("__constant__ double kTabWdot[] = {"
+                + ", ".join(lit(x) for x in tab) + "};")
+        w.lines.insert(table_at, decl)
+        L.tab = None
+
+
+def _emit_wdot_jac(w: _Writer, L: _Lowering):
+    """omega-dot with dw/dC (dense NS x NS, registers) and dw/dT."""
+    m = L.m
+    w("// omega-dot and its derivatives: J[k][j] = d w_k / d C_j, "
+      "JT[k] = d w_k / d T")
+    w("__device__ __forceinline__ void net_production_rates_jac(const double T, "
+      "const double (&C)[NS], double (&w)[NS], double (&J)[NS][NS], "
+      "double (&JT)[NS]) {")
+    w.ind += 1
+    rev = any(r.reversible for r in m.reactions)
+    if L.species_kc:
+        _kin_prologue(w, L, need_g=False)
+        if rev:
+            L.emit_gibbs_exp(w)
+    else:
+        _kin_prologue(w, L, need_g=rev)
+    if rev:
+        w("double hrt[NS];")
+        w("species_h_rt(T, invT, hrt);")
+    w("double Dm[NS];")
+    w("#pragma unroll")
+    w("for (int k = 0; k < NS; ++k) {")
+    w("  w[k] = 0.0; JT[k] = 0.0; Dm[k] = 0.0;")
+    w("#pragma unroll")
+    w("  for (int j = 0; j < NS; ++j) J[k][j] = 0.0;")
+    w("}")
+    for i, r in enumerate(m.reactions):
+        w(f"{{  // R{i}: {r.equation} [{r.reaction_type}]")
+        w.ind += 1
+        L.kf_block_deriv(w, r)
+        rpairs = L.reactant_pairs(r)
+        w(f"const double pf = {L.prod_expr(rpairs)};")
+        dpf = dict(L.prod_partials(rpairs))
+        dpb = {}
+        if r.reversible:
+            ppairs = [(L.idx[sp], nu) for sp, nu in r.products]
+            w(f"const double pb = {L.prod_expr(ppairs)};")
+            dpb = dict(L.prod_partials(ppairs))
+            kinv = L.kinv_expr(r) if L.species_kc else f"exp({L.nlk_expr(r)})"
+            w(f"const double kinv = {kinv};")
+            dnu = sum(nu for _, nu in r.products) - sum(nu for _, nu in r.reactants)
+            hs = [f"{lit(-nu)} * hrt[{L.idx[sp]}]" for sp, nu in r.products]
+            hs += [f"{lit(nu)} * hrt[{L.idx[sp]}]" for sp, nu in r.reactants]
+            w(f"const double dkinvT = kinv * invT * ({lit(dnu)} + "
+              + " + ".join(hs) + ");")
+            w("const double delta = fma(-kinv, pb, pf);")
+            w("const double dqT = fma(dkfT, delta, -kf * dkinvT * pb);")
+        else:
+            w("const double delta = pf;")
+            w("const double dqT = dkfT * delta;")
+        w("const double q = kf * delta;")
+        mdep = r.kind in ("three-body", "falloff")
+        if mdep:
+            w("const double dqM = dkfM * delta;")
+        cols = sorted(set(dpf) | set(dpb))
+        for j in cols:
+            a = dpf.get(j)
+            b = dpb.get(j)
+            if a is not None and b is not None:
+                w(f"const double dq{j} = kf * fma(-kinv, {b}, {a});")
+            elif a is not None:
+                w(f"const double dq{j} = kf * ({a});")
+            else:
+                w(f"const double dq{j} = -kf * kinv * ({b});")
+        eff = []
+        if mdep:
+            d = r.default_efficiency
+            eff = [(L.idx[sp], e - d) for sp, e in r.efficiencies if e - d != 0.0]
+        for k, nu in L.species_updates(r, "q"):
+            nl = lit(nu)
+            w(f"w[{k}] = fma({nl}, q, w[{k}]);")
+            w(f"JT[{k}] = fma({nl}, dqT, JT[{k}]);")
+            for j in cols:
+                w(f"J[{k}][{j}] = fma({nl}, dq{j}, J[{k}][{j}]);")
+            if mdep:
+                for j, e in eff:
+                    w(f"J[{k}][{j}] = fma({lit(nu * e)}, dqM, J[{k}][{j}]);")
+                if r.default_efficiency != 0.0:
+                    w(f"Dm[{k}] = fma({lit(nu * r.default_efficiency)}, dqM, Dm[{k}]);")
+        w.ind -= 1
+        w("}")
+    w("#pragma unroll")
+    w("for (int k = 0; k < NS; ++k)")
+    w("#pragma unroll")
+    w("  for (int j = 0; j < NS; ++j) J[k][j] += Dm[k];")
     w.ind -= 1
     w("}")
     w("")
@@ -445,6 +791,9 @@ def _emit_creation_destruction(w: _Writer, L: _Lowering):
 class HipCodeGenerator(CodeGenerator):
     """``hip`` target: mechanism IR -> HIP C++ translation unit."""
 
+    #: overrides of DEFAULT_TUNING (class-level; set before generate())
+    tuning: Optional[dict] = None
+
     @staticmethod
     def get_name() -> str:
         return "hip"
@@ -465,7 +814,7 @@ class HipCodeGenerator(CodeGenerator):
         if opts.scalar_type not in (None, "double"):
             raise TypeError("the hip target computes in double precision only")
         mech = sol
-        L = _Lowering(mech)
+        L = _Lowering(mech, HipCodeGenerator.tuning)
         w = _Writer()
         wdot_block = 256
         _emit_prelude(w, L, name, mech.digest(), wdot_block)
@@ -473,9 +822,14 @@ class HipCodeGenerator(CodeGenerator):
         _emit_wdot(w, L)
         _emit_reaction_quantities(w, L)
         _emit_creation_destruction(w, L)
+        dense = mech.n_species <= DENSE_JAC_MAX_SPECIES
+        if dense:
+            _emit_wdot_jac(w, L)
         w("}  // namespace mech")
         w("")
         w('#include "pyro_thermo_kernels.hpp"')
+        if dense:
+            w('#include "pyro_flamelet_kernels.hpp"')
         return w.text()
 
     @staticmethod

@@ -1,0 +1,189 @@
+// Batched block-tridiagonal solver (block Thomas algorithm) for gfx950.
+//
+// Solves, for each system s of a batch with ragged sizes N_s,
+//   C_0 x_0 + U_0 x_1 = r_0
+//   L_{i-1} x_{i-1} + C_i x_i + U_i x_{i+1} = r_i      (0 < i < N-1)
+//   L_{N-2} x_{N-2} + C_{N-1} x_{N-1} = r_{N-1}
+// with NV x NV blocks: the algorithm of pyrometheus/flamelets/
+// linear_solver.py:6-103 (forward elimination C'_i = C_i - L_{i-1} C'^-1 U,
+// r'_i = r_i - L_{i-1} C'^-1 r', then back substitution).
+//
+// Mapping: one wave64 per system.  Lane l (< NV) owns one logical row of the
+// augmented block [C' | U | r'] (2 NV + 1 doubles in registers).  Each block
+// row is reduced by Gauss-Jordan elimination with partial pivoting in which
+// the pivot row is never moved: the lane chosen as pivot for column k simply
+// takes logical row k ("virtual pivoting"), the pivot row is broadcast with
+// v_readlane, and every other lane eliminates with its own column-k entry.
+// The result [I | C'^-1 U | C'^-1 r'] is stored to a workspace for the back
+// substitution, and each lane builds its logical row of the next block from
+// the row it already holds, so no row data ever crosses lanes.  With DIAG the
+// off-diagonal blocks are scalar multiples of the identity (the flamelet
+// Laplacian coupling), given as one scalar per block.
+//
+// Workspace: for global block row g, rows k=0..NV-1 of
+// [C'^-1 U | C'^-1 r'] at work[(g*NV + k)*(NV+1) + c].
+#pragma once
+
+#include "pyro_common.hpp"
+
+namespace pyro_bt {
+
+__device__ __forceinline__ double bcast(double v, int lane) {
+  // lane must be wave-uniform
+  const int lo = __builtin_amdgcn_readlane(
+      static_cast<int>(__double_as_longlong(v) & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane(
+      static_cast<int>(__double_as_longlong(v) >> 32), lane);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
+// Gauss-Jordan on the augmented rows a[0..W) held one per lane (lanes < NV).
+// On return the lane's row is logical row `my_k` of [I | X].  `perm[k]` is
+// the lane holding logical row k (wave-uniform).
+template <int NV, int W>
+__device__ __forceinline__ void gauss_jordan(double (&a)[W], int lane,
+                                             int& my_k, int (&perm)[NV],
+                                             bool& singular) {
+  unsigned long long used = 0ull;
+  my_k = -1;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    // partial pivoting: largest |a[k]| among lanes not yet used
+    double best = -1.0;
+    int p = -1;
+#pragma unroll
+    for (int r = 0; r < NV; ++r) {
+      const double v = bcast(fabs(a[k]), r);
+      const bool free_r = ((used >> r) & 1ull) == 0ull;
+      if (free_r && (p < 0 || v > best)) {  // NaN rows still get a pivot slot
+        best = v;
+        p = r;
+      }
+    }
+    p = __builtin_amdgcn_readfirstlane(p);
+    used |= 1ull << p;
+    perm[k] = p;
+    const double piv = bcast(a[k], p);
+    singular = singular || !(fabs(piv) > 0.0);
+    const double inv = 1.0 / piv;
+    const bool is_p = (lane == p);
+    const double f = a[k];
+#pragma unroll
+    for (int j = k + 1; j < W; ++j) {
+      const double prow = bcast(a[j], p) * inv;
+      a[j] = is_p ? prow : fma(-f, prow, a[j]);
+    }
+    a[k] = is_p ? 1.0 : 0.0;
+    if (is_p) my_k = k;
+  }
+}
+
+template <int NV, bool DIAG>
+__global__ void __launch_bounds__(64)
+k_block_thomas(const double* __restrict__ L, const double* __restrict__ C,
+               const double* __restrict__ U, const double* __restrict__ R,
+               double* __restrict__ X, const int64_t* __restrict__ offsets,
+               double* __restrict__ work, int* __restrict__ n_singular) {
+  constexpr int W = 2 * NV + 1;
+  const int s = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t off = offsets[s];
+  const int64_t n = offsets[s + 1] - off;
+  if (n <= 0) return;
+  const bool active = lane < NV;
+  bool singular = false;
+
+  double a[W];
+  int perm[NV];
+  int my_k = lane;
+  // block row 0: [C_0 | U_0 | r_0], logical row = lane
+  {
+    const int64_t g = off;
+    const int r = active ? lane : 0;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a[c] = C[(g * NV + r) * NV + c];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      if constexpr (DIAG) {
+        a[NV + c] = (n > 1 && c == r) ? U[g] : 0.0;
+      } else {
+        a[NV + c] = n > 1 ? U[(g * NV + r) * NV + c] : 0.0;
+      }
+    }
+    a[2 * NV] = R[g * NV + r];
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t g = off + i;
+    gauss_jordan<NV, W>(a, lane, my_k, perm, singular);
+    // store logical row my_k of [C'^-1 U | C'^-1 r']
+    if (active) {
+      double* wrow = work + (g * NV + my_k) * (NV + 1);
+#pragma unroll
+      for (int c = 0; c <= NV; ++c) wrow[c] = a[NV + c];
+    }
+    if (i + 1 == n) break;
+    // next block row, logical row my_k:
+    //   C'_{i+1}[k,:] = C_{i+1}[k,:] - (L_i X_U)[k,:]
+    //   r'_{i+1}[k]   = r_{i+1}[k]   - (L_i X_r)[k]
+    const int64_t g1 = g + 1;
+    const int k = active ? my_k : 0;
+    double nxt[W];
+    if constexpr (DIAG) {
+      const double l = L[g];
+#pragma unroll
+      for (int c = 0; c < NV; ++c)
+        nxt[c] = fma(-l, a[NV + c], C[(g1 * NV + k) * NV + c]);
+      nxt[2 * NV] = fma(-l, a[2 * NV], R[g1 * NV + k]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < NV; ++c) nxt[c] = C[(g1 * NV + k) * NV + c];
+      nxt[2 * NV] = R[g1 * NV + k];
+#pragma unroll
+      for (int m = 0; m < NV; ++m) {
+        const double lkm = L[(g * NV + k) * NV + m];
+        const int src = perm[m];  // lane holding logical row m
+#pragma unroll
+        for (int c = 0; c < NV; ++c)
+          nxt[c] = fma(-lkm, bcast(a[NV + c], src), nxt[c]);
+        nxt[2 * NV] = fma(-lkm, bcast(a[2 * NV], src), nxt[2 * NV]);
+      }
+    }
+    const bool has_u = (i + 2 < n);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      if constexpr (DIAG) {
+        nxt[NV + c] = (has_u && c == k) ? U[g1] : 0.0;
+      } else {
+        nxt[NV + c] = has_u ? U[(g1 * NV + k) * NV + c] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < W; ++c) a[c] = nxt[c];
+  }
+  // the workspace rows were written by other lanes of this wave
+  __syncthreads();
+  // back substitution, lane = logical row: x_i = X_r,i - X_U,i x_{i+1}
+  const int r = active ? lane : 0;
+  double x_next[NV];
+  {
+    const int64_t g = off + n - 1;
+    const double xr = work[(g * NV + r) * (NV + 1) + NV];
+    if (active) X[g * NV + r] = xr;
+#pragma unroll
+    for (int m = 0; m < NV; ++m) x_next[m] = bcast(xr, m);
+  }
+  for (int64_t i = n - 2; i >= 0; --i) {
+    const int64_t g = off + i;
+    const double* wrow = work + (g * NV + r) * (NV + 1);
+    double xr = wrow[NV];
+#pragma unroll
+    for (int m = 0; m < NV; ++m) xr = fma(-wrow[m], x_next[m], xr);
+    if (active) X[g * NV + r] = xr;
+#pragma unroll
+    for (int m = 0; m < NV; ++m) x_next[m] = bcast(xr, m);
+  }
+  if (singular && lane == 0 && n_singular) atomicAdd(n_singular, 1);
+}
+
+}  // namespace pyro_bt

@@ -51,4 +51,18 @@ __device__ __forceinline__ double ipow(double x) {
   }
 }
 
+// Pointer into the constant address space: uniform reads through it become
+// scalar loads (one s_load_dwordx16 fetches eight doubles into SGPRs).
+typedef const __attribute__((address_space(4))) double* cptr;
+
+// Reciprocal from the hardware estimate plus two Newton steps (within an
+// ulp or so of 1/x for the positive normal operands it is used on).
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
 }  // namespace pyro

@@ -116,7 +116,10 @@ def build_mechanism_library(name_or_mech, source: Optional[str] = None,
 
 
 def core_lib_path() -> str:
-    return os.path.join(GEN_DIR, "libpyro_core.so")
+    import hashlib
+    with open(os.path.join(CSRC_DIR, "pyro_core.hip"), "rb") as f:
+        d = hashlib.sha256(f.read() + source_digest().encode()).hexdigest()[:8]
+    return os.path.join(GEN_DIR, f"libpyro_core_{d}.so")
 
 
 def build_core_library(force: bool = False) -> str:
@@ -124,10 +127,7 @@ def build_core_library(force: bool = False) -> str:
     so = core_lib_path()
     src = os.path.join(CSRC_DIR, "pyro_core.hip")
     if os.path.exists(so) and not force:
-        if os.path.getmtime(so) >= max(
-                os.path.getmtime(os.path.join(CSRC_DIR, f))
-                for f in os.listdir(CSRC_DIR)):
-            return so
+        return so
     compile_hip(src, so)
     return so
 
@@ -214,8 +214,9 @@ def load_mechanism_library(mech: Mechanism, build: bool = True) -> HipLibrary:
 
 
 CORE_SIGNATURES = {
+    "pyro_block_thomas_supported": (_I,),
     "pyro_block_thomas_batched": (_P, _P, _P, _P, _P, _P, _I64, _I, _I, _P,
-                                  _P),
+                                  _P, _P),
 }
 
 

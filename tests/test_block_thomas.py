@@ -1,0 +1,106 @@
+"""Block-Thomas: the oracle against dense solves (CPU) and the batched HIP
+kernel against the oracle (GPU).  Tolerance: relative 1e-12 on well-
+conditioned systems (block-diagonally dominant or pivoting-stressed)."""
+import numpy as np
+import pytest
+
+from oracle.flamelet_np import assemble_dense, block_thomas
+
+
+def _system(rng, n, nv, dominant=True, pivot_stress=False):
+    L = rng.normal(size=(max(n - 1, 0), nv, nv))
+    U = rng.normal(size=(max(n - 1, 0), nv, nv))
+    C = rng.normal(size=(n, nv, nv))
+    if dominant:
+        C += 4 * nv * np.eye(nv)
+    if pivot_stress:
+        # zero diagonal entries: elimination without row exchanges fails
+        for i in range(n):
+            C[i][np.arange(nv), np.arange(nv)] = 0.0
+            C[i] += 4 * nv * np.roll(np.eye(nv), 1, axis=1)
+    r = rng.normal(size=(n, nv))
+    return L, C, U, r
+
+
+@pytest.mark.parametrize("nv", [1, 3, 10])
+def test_oracle_matches_dense_solve(nv):
+    rng = np.random.default_rng(nv)
+    for n in (1, 2, 9):
+        L, C, U, r = _system(rng, n, nv)
+        x = block_thomas(L, C, U, r)
+        A = assemble_dense(L, C, U) if n > 1 else C[0]
+        xd = np.linalg.solve(A, r.ravel())
+        assert np.max(np.abs(x.ravel() - xd)) < 1e-12 * np.max(np.abs(xd))
+
+
+def test_oracle_scalar_thomas():
+    """nv = 1 reduces to the textbook Thomas algorithm."""
+    rng = np.random.default_rng(0)
+    n = 12
+    a, c = rng.uniform(-1, 1, n - 1), rng.uniform(-1, 1, n - 1)
+    b = rng.uniform(3, 4, n)
+    d = rng.normal(size=n)
+    cp, dp = np.zeros(n), np.zeros(n)
+    cp[0], dp[0] = c[0] / b[0], d[0] / b[0]
+    for i in range(1, n):
+        m = b[i] - a[i - 1] * cp[i - 1]
+        cp[i] = c[i] / m if i < n - 1 else 0.0
+        dp[i] = (d[i] - a[i - 1] * dp[i - 1]) / m
+    x = np.zeros(n)
+    x[-1] = dp[-1]
+    for i in range(n - 2, -1, -1):
+        x[i] = dp[i] - cp[i] * x[i + 1]
+    got = block_thomas(a.reshape(-1, 1, 1), b.reshape(-1, 1, 1),
+                       c.reshape(-1, 1, 1), d.reshape(-1, 1))
+    assert np.max(np.abs(got[:, 0] - x)) < 1e-14
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nv", [1, 2, 5, 9, 10, 16, 20])
+@pytest.mark.parametrize("diag", [False, True])
+def test_gpu_batched_ragged(nv, diag, device):
+    import torch
+    from pyrometheus_amd.flamelets.linear_solver import block_thomas_batched
+    rng = np.random.default_rng(100 + nv)
+    sizes = [1, 2, 3, 17, 64, 5]
+    systems = []
+    for j, n in enumerate(sizes):
+        L, C, U, r = _system(rng, n, nv, pivot_stress=(j % 2 == 1 and nv > 1))
+        if diag:
+            lv, uv = rng.normal(size=max(n - 1, 0)), rng.normal(size=max(n - 1, 0))
+            L = lv[:, None, None] * np.eye(nv)
+            U = uv[:, None, None] * np.eye(nv)
+        systems.append((L, C, U, r))
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    P = offs[-1]
+    if diag:
+        Lb, Ub = np.zeros(P), np.zeros(P)
+    else:
+        Lb, Ub = np.zeros((P, nv, nv)), np.zeros((P, nv, nv))
+    Cb, Rb = np.zeros((P, nv, nv)), np.zeros((P, nv))
+    for (L, C, U, r), o, n in zip(systems, offs, sizes):
+        Cb[o:o + n], Rb[o:o + n] = C, r
+        if n > 1:
+            if diag:
+                Lb[o:o + n - 1], Ub[o:o + n - 1] = L[:, 0, 0], U[:, 0, 0]
+            else:
+                Lb[o:o + n - 1], Ub[o:o + n - 1] = L, U
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=device)
+    x = block_thomas_batched(t(Lb), t(Cb), t(Ub), t(Rb),
+                             torch.tensor(offs, device=device), diag=diag,
+                             check_singular=True).cpu().numpy()
+    for (L, C, U, r), o, n in zip(systems, offs, sizes):
+        ref = block_thomas(L, C, U, r)
+        err = np.max(np.abs(x[o:o + n] - ref)) / np.max(np.abs(ref))
+        assert err < 1e-12, (n, err)
+
+
+@pytest.mark.gpu
+def test_gpu_drop_in_signature(device):
+    import torch
+    from pyrometheus_amd.flamelets.linear_solver import block_thomas as bt
+    rng = np.random.default_rng(5)
+    L, C, U, r = _system(rng, 33, 10)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=device)
+    x = bt(t(L), t(C), t(U), t(r)).cpu().numpy()
+    assert np.max(np.abs(x - block_thomas(L, C, U, r))) < 1e-12 * np.max(np.abs(x))
