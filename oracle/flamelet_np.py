@@ -355,6 +355,24 @@ class FlameletSolver:
             self.log.append(("cn", step, err))
         return phi, temp
 
+    def equilibrate(self, phi, t_guess, p, h_ox, h_fu, steps=24, dt0=1e-8,
+                    growth=2.5, newton_maxiter=20, newton_tol=1e-9):
+        """Crank-Nicolson steps with zero interior dissipation and a
+        geometrically growing step: relaxes each interior point to chemical
+        equilibrium at its (h, p) (the device solver's ``equilibrate``)."""
+        a, b, c = self.eqns.abc
+        chi0 = np.zeros(len(self.eqns.x))
+        chi0[0], chi0[-1] = 2.0 / b[0], 2.0 / b[-1]
+        vd = np.zeros(len(self.eqns.x))
+        dt = dt0
+        for _ in range(steps):
+            phi, it, err = self.flamelet_time_step(
+                phi, newton_maxiter, newton_tol, phi, dt, chi0, vd, t_guess, p,
+                h_ox, h_fu)
+            dt *= growth
+        T = temperature_from_enthalpy(self.eqns.gas, phi[0], phi[1:], t_guess)
+        return phi, T
+
     def solve(self, newton_maxiter, newton_tol, bdf_newton_maxiter,
               bdf_newton_tol, bdf_time_step, bdf_maxsteps, try_newton,
               max_attempts, chi, vd, t_guess, p, h_ox, h_fu, phi_guess):

@@ -1,0 +1,158 @@
+"""Device engine shared by the plain and batched flamelet solvers.
+
+A :class:`FlameletBatch` holds B flamelets whose grid points are concatenated
+(P = sum N_j points): state ``phi`` (nv, P) with row 0 the enthalpy, the
+per-point Laplacian coefficients and dissipation rate, and the scratch the
+kernels need.  One Newton update for every flamelet of the batch is one
+launch sequence:
+
+1. ``pyro_flamelet_source`` -- per point: T(h, Y) by Newton, rho, S and the
+   (nv x nv) source Jacobian (chain rule through the analytic kinetics
+   Jacobian);
+2. ``pyro_flamelet_assemble`` -- residual and block-tridiagonal Jacobian,
+   steady or Crank-Nicolson per flamelet;
+3. ``pyro_flamelet_thomas`` -- block Thomas, one wave per flamelet, with the
+   Laplacian coupling blocks as scalar multiples of the identity;
+4. ``pyro_flamelet_update`` -- phi += v for the selected flamelets and the
+   per-flamelet sum of squares of v (fixed-order reduction, so a flamelet's
+   result does not depend on what else is in the batch).
+"""
+
+from __future__ import annotations
+
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from ..runtime import HipError
+from .domain import laplacian_coefficients
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+class FlameletBatch:
+    def __init__(self, gas, grids: Sequence[np.ndarray], y_ox, y_fu,
+                 device=None):
+        lib = gas._lib
+        if not lib.has("pyro_flamelet_source"):
+            raise HipError(f"mechanism {gas._mech.name} has no flamelet "
+                           "kernels (more than 16 species)")
+        self.gas, self.lib = gas, lib
+        self.dev = torch.device(device) if device is not None else gas.device
+        self.ns = gas.num_species
+        self.nv = self.ns + 1
+        self.sizes = [len(x) for x in grids]
+        if min(self.sizes) < 3:
+            raise ValueError("each flamelet needs at least 3 grid points")
+        self.B = len(self.sizes)
+        offs = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.offsets_np = offs
+        self.P = int(offs[-1])
+        dd = dict(dtype=torch.float64, device=self.dev)
+        abc = [laplacian_coefficients(np.asarray(x, dtype=float)) for x in grids]
+        self.la = torch.tensor(np.concatenate([a for a, _, _ in abc]), **dd)
+        self.lb = torch.tensor(np.concatenate([b for _, b, _ in abc]), **dd)
+        self.lc = torch.tensor(np.concatenate([c for _, _, c in abc]), **dd)
+        self.off = torch.tensor(offs, dtype=torch.int64, device=self.dev)
+        self.sys = torch.tensor(np.repeat(np.arange(self.B), self.sizes),
+                                dtype=torch.int32, device=self.dev)
+        y_ox = np.broadcast_to(np.asarray(y_ox, dtype=float), (self.B, self.ns))
+        y_fu = np.broadcast_to(np.asarray(y_fu, dtype=float), (self.B, self.ns))
+        self.y_ox, self.y_fu = y_ox.copy(), y_fu.copy()
+        self.bc = torch.zeros((self.B, 2, self.nv), **dd)
+        self.bc[:, 0, 1:] = torch.tensor(self.y_ox, **dd)
+        self.bc[:, 1, 1:] = torch.tensor(self.y_fu, **dd)
+        P, nv = self.P, self.nv
+        self.S = torch.empty((nv, P), **dd)
+        self.J = torch.empty((P, nv, nv), **dd)
+        self.rhs = torch.empty((P, nv), **dd)
+        self.x = torch.empty((P, nv), **dd)
+        self.L = torch.empty(P, **dd)
+        self.U = torch.empty(P, **dd)
+        self.work = torch.empty(P * nv * (nv + 1), **dd)
+        self.norm2 = torch.empty(self.B, **dd)
+        self.T = torch.empty(P, **dd)
+        self.n_fail = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        self.modes = torch.zeros(self.B, dtype=torch.int32, device=self.dev)
+        self.dt = torch.zeros(self.B, **dd)
+        self.active = torch.ones(self.B, dtype=torch.int32, device=self.dev)
+        self.none_active = torch.zeros(self.B, dtype=torch.int32,
+                                       device=self.dev)
+
+    # -- plumbing --------------------------------------------------------------
+    def per_point(self, values) -> torch.Tensor:
+        """Expand one value per flamelet to one per point."""
+        v = torch.as_tensor(np.asarray(values, dtype=float), dtype=torch.float64)
+        v = torch.broadcast_to(v, (self.B,)).to(self.dev)
+        return torch.repeat_interleave(v, torch.tensor(self.sizes,
+                                                       device=self.dev))
+
+    def set_boundary_enthalpy(self, h_ox, h_fu):
+        dd = dict(dtype=torch.float64, device=self.dev)
+        self.bc[:, 0, 0] = torch.as_tensor(
+            np.broadcast_to(np.asarray(h_ox, dtype=float), (self.B,)).copy(), **dd)
+        self.bc[:, 1, 0] = torch.as_tensor(
+            np.broadcast_to(np.asarray(h_fu, dtype=float), (self.B,)).copy(), **dd)
+
+    def split(self, t: torch.Tensor) -> List[torch.Tensor]:
+        return [t[..., a:b] for a, b in zip(self.offsets_np[:-1],
+                                            self.offsets_np[1:])]
+
+    # -- kernels ---------------------------------------------------------------
+    def source(self, phi, vd, tguess, p, jac: bool, tol=1e-10, maxit=100):
+        self.n_fail.zero_()
+        self.lib.call("pyro_flamelet_source", phi.data_ptr(), phi.stride(0),
+                      _ptr(vd), tguess.data_ptr(), 1, p.data_ptr(), 1,
+                      self.S.data_ptr(), self.P, self.T.data_ptr(),
+                      self.J.data_ptr() if jac else None, self.P, tol, maxit,
+                      self.n_fail.data_ptr(), _stream())
+
+    def assemble(self, phi, chi, modes: Optional[torch.Tensor] = None,
+                 phi_prev=None, r_prev=None, r_out=None):
+        self.lib.call("pyro_flamelet_assemble", phi.data_ptr(), phi.stride(0),
+                      self.S.data_ptr(), self.P, self.J.data_ptr(),
+                      chi.data_ptr(), self.la.data_ptr(), self.lb.data_ptr(),
+                      self.lc.data_ptr(), self.sys.data_ptr(),
+                      self.off.data_ptr(), self.bc.data_ptr(), _ptr(modes),
+                      _ptr(phi_prev), _ptr(r_prev),
+                      self.dt.data_ptr() if modes is not None else None,
+                      self.rhs.data_ptr(), self.L.data_ptr(),
+                      self.U.data_ptr(), _ptr(r_out), self.P, _stream())
+
+    def thomas(self):
+        self.lib.call("pyro_flamelet_thomas", self.L.data_ptr(),
+                      self.J.data_ptr(), self.U.data_ptr(),
+                      self.rhs.data_ptr(), self.x.data_ptr(),
+                      self.off.data_ptr(), self.B, self.work.data_ptr(), None,
+                      _stream())
+
+    def update(self, phi, active: Optional[torch.Tensor]):
+        self.lib.call("pyro_flamelet_update", phi.data_ptr(), phi.stride(0),
+                      self.x.data_ptr(), self.sys.data_ptr(), _ptr(active),
+                      self.off.data_ptr(), self.B, self.norm2.data_ptr(),
+                      self.P, _stream())
+
+    def residual(self, phi, chi, vd, tguess, p, out):
+        """Steady residual R(phi) into ``out`` (nv, P).  The block storage is
+        overwritten with values that are not used."""
+        self.source(phi, vd, tguess, p, jac=False)
+        self.assemble(phi, chi, r_out=out)
+        return out
+
+    def newton_update(self, phi, chi, vd, tguess, p, modes=None,
+                      phi_prev=None, r_prev=None, apply=True, active=None):
+        """One Newton update v = -J^-1 R for every flamelet (steady, or
+        Crank-Nicolson where ``modes`` is 1).  Applies it when ``apply``;
+        returns the per-flamelet |v|^2 (device) and v (P, nv)."""
+        self.source(phi, vd, tguess, p, jac=True)
+        self.assemble(phi, chi, modes, phi_prev, r_prev)
+        self.thomas()
+        self.update(phi, active if apply else self.none_active)
+        return self.norm2, self.x

@@ -1,0 +1,460 @@
+"""Steady flamelet equations and solvers on the device.
+
+``FlameletEquations`` and ``FlameletSolver`` keep the reference's names,
+argument orders and control flow (``pyrometheus/flamelets/equations.py``,
+``solver.py:16-600``): a host loop per Newton iteration that reads back the
+update norm, a Crank-Nicolson time march as globalisation, and the hybrid
+``solve`` with its restart attempts.  Every evaluation (temperature recovery,
+source terms and their Jacobian, assembly, block-Thomas solve, update) is a
+HIP kernel; tensors are float64 on the ROCm device.
+
+``solve_batch`` solves many flamelets (uneven grid sizes, different
+dissipation rates, pressures or boundary states) at once: every flamelet runs
+the same state machine as ``solve`` on its own, and one launch sequence per
+iteration advances all unfinished flamelets, whichever phase each is in.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from .engine import FlameletBatch
+from .state import FlameletState, _array_to_state, _state_to_array
+
+
+def _t(x, dev, n=None):
+    if isinstance(x, torch.Tensor):
+        t = x.to(device=dev, dtype=torch.float64)
+    else:
+        t = torch.as_tensor(np.asarray(x, dtype=float), dtype=torch.float64,
+                            device=dev)
+    if n is not None and t.numel() == 1:
+        t = t.reshape(1).expand(n)
+    return t.contiguous()
+
+
+class FlameletEquations:
+    """Residual and block Jacobian of the steady flamelet equations."""
+
+    def __init__(self, pyro_gas, domain, y_ox, y_fu):
+        self.pyro_gas = pyro_gas
+        self.domain = domain
+        self.mol_wts = pyro_gas.molecular_weights
+        self.batch = FlameletBatch(pyro_gas, [domain.x],
+                                   np.asarray(_cpu(y_ox)),
+                                   np.asarray(_cpu(y_fu)))
+        self.num_v = pyro_gas.num_species + 1
+
+    def _args(self, state, diss_rate, viscous_diss, temp_guess, pressure):
+        b, dev = self.batch, self.batch.dev
+        phi = _state_to_array(state).to(dev).contiguous()
+        n = b.P
+        chi = _t(diss_rate, dev, n)
+        vd = _t(viscous_diss, dev, n)
+        tg = _t(temp_guess, dev, n)
+        p = _t(pressure, dev, n)
+        return phi, chi, vd, tg, p
+
+    def boundary_conditions(self, h_ox, h_fu):
+        n, ns = self.batch.P, self.pyro_gas.num_species
+        dev = self.batch.dev
+        h = torch.zeros(n, dtype=torch.float64, device=dev)
+        y = torch.zeros((ns, n), dtype=torch.float64, device=dev)
+        h[0], h[-1] = float(h_ox), float(h_fu)
+        y[:, 0] = torch.as_tensor(self.batch.y_ox[0], dtype=torch.float64,
+                                  device=dev)
+        y[:, -1] = torch.as_tensor(self.batch.y_fu[0], dtype=torch.float64,
+                                   device=dev)
+        return FlameletState(h, y)
+
+    def equation_of_state(self, state, temp_guess, pressure):
+        phi, _, vd, tg, p = self._args(state, 0.0, 0.0, temp_guess, pressure)
+        self.batch.source(phi, vd, tg, p, jac=False)
+        T = self.batch.T.clone()
+        rho = self.pyro_gas.get_density(p, T, phi[1:])
+        return rho, T
+
+    def source_terms(self, state, viscous_diss, temp_guess, pressure):
+        phi, _, vd, tg, p = self._args(state, 0.0, viscous_diss, temp_guess,
+                                       pressure)
+        self.batch.source(phi, vd, tg, p, jac=False)
+        return _array_to_state(self.batch.S.clone())
+
+    def source_jacobian(self, state, viscous_diss, temp_guess, pressure):
+        """dS/dphi per point (N, nv, nv) (``vmap(jacfwd(local_source_terms))``,
+        equations.py:124-129)."""
+        phi, _, vd, tg, p = self._args(state, 0.0, viscous_diss, temp_guess,
+                                       pressure)
+        self.batch.source(phi, vd, tg, p, jac=True)
+        return self.batch.J.clone()
+
+    def rhs(self, state, diss_rate, viscous_diss, temp_guess, pressure, h_ox,
+            h_fu):
+        phi, chi, vd, tg, p = self._args(state, diss_rate, viscous_diss,
+                                         temp_guess, pressure)
+        self.batch.set_boundary_enthalpy(h_ox, h_fu)
+        out = torch.empty_like(phi)
+        self.batch.residual(phi, chi, vd, tg, p, out)
+        return _array_to_state(out)
+
+    def jac(self, state, diss_rate, viscous_diss, temp_guess, pressure):
+        """(lower (N-1,nv,nv), central (N,nv,nv), upper (N-1,nv,nv))."""
+        phi, chi, vd, tg, p = self._args(state, diss_rate, viscous_diss,
+                                         temp_guess, pressure)
+        b = self.batch
+        b.source(phi, vd, tg, p, jac=True)
+        b.assemble(phi, chi)
+        eye = torch.eye(self.num_v, dtype=torch.float64, device=b.dev)
+        lower = b.L[:-1, None, None] * eye
+        upper = b.U[:-1, None, None] * eye
+        return lower, b.J.clone(), upper
+
+
+def _cpu(x):
+    return x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else x
+
+
+class FlameletSolver:
+    """Plain drop-in: one flamelet, host loop per Newton iteration."""
+
+    def __init__(self, domain, pyro_gas, mass_frac_bcs):
+        y_ox, y_fu = mass_frac_bcs
+        self.domain = domain
+        self.gov_eqns = FlameletEquations(pyro_gas, domain, y_ox, y_fu)
+        self.num_v = pyro_gas.num_species + 1
+        self.log = []
+
+    # one Newton update v = -J^-1 R (solver.py:63-101)
+    def _update(self, state, diss_rate, viscous_diss, temp_guess, pressure,
+                h_ox, h_fu, mode=0, state_prev=None, time_step=0.0):
+        eq = self.gov_eqns
+        b = eq.batch
+        phi, chi, vd, tg, p = eq._args(state, diss_rate, viscous_diss,
+                                       temp_guess, pressure)
+        b.set_boundary_enthalpy(h_ox, h_fu)
+        if mode == 1:
+            phi_prev = _state_to_array(state_prev).to(b.dev).contiguous()
+            r_prev = torch.empty_like(phi_prev)
+            b.residual(phi_prev, chi, vd, tg, p, r_prev)
+            b.modes.fill_(1)
+            b.dt.fill_(float(time_step))
+            b.newton_update(phi, chi, vd, tg, p, b.modes, phi_prev, r_prev,
+                            apply=False)
+        else:
+            b.newton_update(phi, chi, vd, tg, p, apply=False)
+        v = b.x.T.contiguous()
+        return _array_to_state(v), float(b.norm2[0].item()) ** 0.5
+
+    def flamelet_newton_step(self, state_guess, diss_rate, viscous_diss,
+                             temp_guess, pressure, h_ox, h_fu):
+        return self._update(state_guess, diss_rate, viscous_diss, temp_guess,
+                            pressure, h_ox, h_fu)[0]
+
+    def _newton_loop_py(self, verbosity, state, maxiter, tol, *args):
+        err_prev = np.inf
+        delta = np.inf
+        it = 0
+        for it in range(maxiter):
+            t0 = time.time()
+            v, delta = self._update(state, *args)
+            state = state + v
+            self.log.append(("newton", it, delta))
+            if verbosity:
+                print(f"Newton iter {it}: time: {(time.time() - t0):.4e} s, "
+                      f"|v| = {delta:.4e}")
+            if delta < tol:
+                return state, it, delta, True
+            if err_prev < delta:
+                return state, it, delta, False
+            err_prev = delta
+        return state, it, delta, False
+
+    def flamelet_time_step(self, state, newton_maxiter, newton_tol, state_prev,
+                           time_step, diss_rate, viscous_diss, temp_guess,
+                           pressure, h_ox, h_fu):
+        it, delta = 0, np.inf
+        while delta > newton_tol and it < newton_maxiter:
+            v, delta = self._update(state, diss_rate, viscous_diss, temp_guess,
+                                    pressure, h_ox, h_fu, mode=1,
+                                    state_prev=state_prev,
+                                    time_step=time_step)
+            state = state + v
+            it += 1
+        return state, it, delta
+
+    def flamelet_time_march(self, verbosity, newton_maxiter, newton_tol,
+                            maxsteps, time_step, diss_rate, viscous_diss,
+                            temp_guess, pressure, h_ox, h_fu, state_guess):
+        state, temp = state_guess, temp_guess
+        for step in range(maxsteps):
+            state_prev = state
+            t0 = time.time()
+            state, it, err = self.flamelet_time_step(
+                state, newton_maxiter, newton_tol, state_prev, time_step,
+                diss_rate, viscous_diss, temp, pressure, h_ox, h_fu)
+            temp = self.gov_eqns.pyro_gas.get_temperature_from_enthalpy(
+                state.enthalpy, state.mass_fractions,
+                _t(temp_guess, state.enthalpy.device, state.enthalpy.numel()))
+            self.log.append(("cn", step, err))
+            if verbosity:
+                print(f"BDF time step {step}: time: {time.time() - t0:.4e} s, "
+                      f"T_max = {float(temp.max()):.3f} K ")
+        return state, temp
+
+    def equilibrate(self, state, temp_guess, pressure, h_ox, h_fu,
+                    steps=24, dt0=1e-8, growth=2.5, newton_maxiter=20,
+                    newton_tol=1e-9):
+        """Relax each interior point towards chemical equilibrium at its own
+        (h, p): Crank-Nicolson steps of the flamelet equations with the
+        interior dissipation rate set to zero and the step growing
+        geometrically.  Gives the burning initial state that the reference
+        notebook obtains from Cantera's ``equilibrate('HP')``."""
+        n = state.enthalpy.numel()
+        dev = state.enthalpy.device
+        chi0 = torch.zeros(n, dtype=torch.float64, device=dev)
+        e0, e1 = self.domain.boundary_dissipation
+        chi0[0], chi0[-1] = e0, e1
+        vd = torch.zeros(n, dtype=torch.float64, device=dev)
+        dt = dt0
+        for _ in range(steps):
+            state, it, err = self.flamelet_time_step(
+                state, newton_maxiter, newton_tol, state, dt, chi0, vd,
+                temp_guess, pressure, h_ox, h_fu)
+            dt *= growth
+        temp = self.gov_eqns.pyro_gas.get_temperature_from_enthalpy(
+            state.enthalpy, state.mass_fractions, _t(temp_guess, dev, n))
+        return state, temp
+
+    def warmup(self, warmup_fn_name, *args):
+        if warmup_fn_name == "flamelet_newton_step":
+            return self.flamelet_newton_step(*args)
+        if warmup_fn_name == "flamelet_time_step":
+            return self.flamelet_time_step(*args)
+        raise ValueError(f"Unknown warmup function name {warmup_fn_name}")
+
+    def solve(self, verbosity, newton_maxiter, newton_tol, bdf_newton_maxiter,
+              bdf_newton_tol, bdf_time_step, bdf_maxsteps, try_newton,
+              max_attempts, diss_rate, viscous_diss, temp_guess, pressure,
+              h_ox, h_fu, state_guess):
+        """Hybrid Newton / time-march solve (solver.py:476-600); returns
+        ``(state, temperature)`` and never raises on non-convergence."""
+        gas = self.gov_eqns.pyro_gas
+        dev = state_guess.enthalpy.device
+        n = state_guess.enthalpy.numel()
+        self.converged = False
+        if try_newton:
+            state, it, err, ok = self._newton_loop_py(
+                verbosity, state_guess, newton_maxiter, newton_tol, diss_rate,
+                viscous_diss, temp_guess, pressure, h_ox, h_fu)
+            if ok:
+                self.converged = True
+                return state, gas.get_temperature_from_enthalpy(
+                    state.enthalpy, state.mass_fractions,
+                    _t(temp_guess, dev, n))
+        state, temp = state_guess, temp_guess
+        for i in range(max_attempts):
+            if verbosity:
+                print(f"Attempt {i} of BDF time marching")
+            state, temp = self.flamelet_time_march(
+                verbosity, bdf_newton_maxiter, bdf_newton_tol, bdf_maxsteps,
+                bdf_time_step, diss_rate, viscous_diss, temp, pressure, h_ox,
+                h_fu, state)
+            state_t, it, err, ok = self._newton_loop_py(
+                verbosity, state, newton_maxiter, newton_tol, diss_rate,
+                viscous_diss, temp, pressure, h_ox, h_fu)
+            temp_t = gas.get_temperature_from_enthalpy(
+                state_t.enthalpy, state_t.mass_fractions,
+                _t(temp_guess, dev, n))
+            if ok:
+                self.converged = True
+                return state_t, temp_t
+        if verbosity:
+            print("Flamelet solution did not converge")
+        return state, temp
+
+
+# ---------------------------------------------------------------------------
+# batched solve
+# ---------------------------------------------------------------------------
+@dataclass
+class FlameletProblem:
+    """One steady flamelet: grid ``x`` (N,), dissipation ``chi`` (N,) with the
+    identity-row values at the ends, initial state ``phi`` (nv, N) and
+    temperature guess (N,), boundary streams, pressure."""
+
+    x: np.ndarray
+    chi: np.ndarray
+    phi: np.ndarray
+    t_guess: np.ndarray
+    y_ox: np.ndarray
+    y_fu: np.ndarray
+    h_ox: float
+    h_fu: float
+    pressure: float = 101325.0
+    vd: Optional[np.ndarray] = None
+
+
+@dataclass
+class FlameletResult:
+    state: FlameletState
+    temperature: torch.Tensor
+    converged: bool
+    newton_iterations: int
+    time_steps: int
+    attempts: int
+    finished_at: float = 0.0
+
+
+_NEWTON_TRY, _MARCH, _NEWTON, _DONE_OK, _DONE_FAIL, _EQUIL = range(6)
+
+
+def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
+                newton_maxiter=10, newton_tol=1e-8, bdf_newton_maxiter=20,
+                bdf_newton_tol=1e-9, bdf_time_step=1e-5, bdf_maxsteps=10,
+                try_newton=False, max_attempts=10, verbosity=False,
+                equilibrate_steps=0, equilibrate_dt0=1e-8,
+                equilibrate_growth=2.5,
+                batch: Optional[FlameletBatch] = None) -> List[FlameletResult]:
+    """Solve every problem with the control flow of ``FlameletSolver.solve``
+    applied to each flamelet independently (preceded, when
+    ``equilibrate_steps`` > 0, by ``FlameletSolver.equilibrate``); one launch
+    sequence per iteration advances every unfinished flamelet."""
+    t_start = time.time()
+    B = len(problems)
+    b = batch or FlameletBatch(pyro_gas, [p.x for p in problems],
+                               np.stack([p.y_ox for p in problems]),
+                               np.stack([p.y_fu for p in problems]))
+    dev = b.dev
+    cat = lambda xs: torch.as_tensor(np.concatenate(xs, axis=-1),
+                                     dtype=torch.float64, device=dev)
+    phi = cat([np.asarray(p.phi) for p in problems]).contiguous()
+    phi_guess = phi.clone()
+    chi = cat([p.chi for p in problems])
+    vd = cat([p.vd if p.vd is not None else np.zeros(len(p.x))
+              for p in problems])
+    tg = cat([p.t_guess for p in problems])
+    pres = b.per_point([p.pressure for p in problems])
+    b.set_boundary_enthalpy(np.array([p.h_ox for p in problems]),
+                            np.array([p.h_fu for p in problems]))
+    phi_prev = torch.empty_like(phi)
+    r_prev = torch.empty_like(phi)
+    r_tmp = torch.empty_like(phi)
+    phi_save = torch.empty_like(phi)
+    sys = b.sys.long()
+
+    first_phase = _NEWTON_TRY if try_newton else _MARCH
+    phase = np.full(B, _EQUIL if equilibrate_steps > 0 else first_phase)
+    chi0 = chi.clone()
+    interior = torch.ones(b.P, dtype=torch.bool, device=dev)
+    interior[b.off[:-1]] = False
+    interior[b.off[1:] - 1] = False
+    chi0[interior] = 0.0
+    eq_step = np.zeros(B, dtype=int)
+    it = np.zeros(B, dtype=int)
+    err_prev = np.full(B, np.inf)
+    step = np.zeros(B, dtype=int)
+    inner = np.zeros(B, dtype=int)
+    attempt = np.zeros(B, dtype=int)
+    n_newton = np.zeros(B, dtype=int)
+    n_steps = np.zeros(B, dtype=int)
+    finished = np.zeros(B)
+    b.dt.fill_(float(bdf_time_step))
+
+    def pts(mask_sys):
+        m = torch.as_tensor(mask_sys, dtype=torch.bool, device=dev)
+        return m[sys]
+
+    while True:
+        live = (phase != _DONE_OK) & (phase != _DONE_FAIL)
+        if not live.any():
+            break
+        eq = phase == _EQUIL
+        chi_eff = torch.where(pts(eq), chi0, chi) if eq.any() else chi
+        dts = np.where(eq, equilibrate_dt0 * equilibrate_growth ** eq_step,
+                       bdf_time_step)
+        b.dt.copy_(torch.as_tensor(dts, dtype=torch.float64))
+        cn = (phase == _MARCH) | eq
+        # flamelets starting a Crank-Nicolson step: phi_prev, R(phi_prev)
+        starting = cn & (inner == 0)
+        if starting.any():
+            b.residual(phi, chi_eff, vd, tg, pres, r_tmp)
+            pm = pts(starting)
+            phi_prev[:, pm] = phi[:, pm]
+            r_prev[:, pm] = r_tmp[:, pm]
+        modes = torch.as_tensor(cn.astype(np.int32), device=dev)
+        active = torch.as_tensor(live.astype(np.int32), device=dev)
+        norm2, _ = b.newton_update(phi, chi_eff, vd, tg, pres, modes, phi_prev,
+                                   r_prev, apply=True, active=active)
+        # refresh the temperature guess of live flamelets only, so a finished
+        # flamelet's final temperature does not depend on its neighbours
+        tg.copy_(torch.where(pts(live), b.T, tg))
+        delta = np.sqrt(norm2.cpu().numpy())
+        now = time.time() - t_start
+        newton = live & ((phase == _NEWTON_TRY) | (phase == _NEWTON))
+        n_newton += newton
+        ok = newton & (delta < newton_tol)
+        bad = newton & ~ok & ((err_prev < delta) | (it + 1 >= newton_maxiter))
+        cont = newton & ~ok & ~bad
+        phase[ok] = _DONE_OK
+        finished[ok] = now
+        err_prev[cont] = delta[cont]
+        it[cont] += 1
+        # failed first Newton: restart from the guess with a time march
+        f0 = bad & (phase == _NEWTON_TRY)
+        if f0.any():
+            pm = pts(f0)
+            phi[:, pm] = phi_guess[:, pm]
+            phase[f0], step[f0], inner[f0] = _MARCH, 0, 0
+        # failed Newton after a march: back to the marched state
+        f1 = bad & (phase == _NEWTON)
+        if f1.any():
+            pm = pts(f1)
+            phi[:, pm] = phi_save[:, pm]
+            attempt[f1] += 1
+            again = f1 & (attempt < max_attempts)
+            phase[again], step[again], inner[again] = _MARCH, 0, 0
+            give_up = f1 & ~again
+            phase[give_up] = _DONE_FAIL
+            finished[give_up] = now
+        # Crank-Nicolson inner iterations (time march and equilibration)
+        march = live & cn
+        inner[march] += 1
+        step_done = march & ((delta <= bdf_newton_tol)
+                             | (inner >= bdf_newton_maxiter))
+        inner[step_done] = 0
+        eq_done = step_done & eq
+        eq_step[eq_done] += 1
+        eq_finished = eq_done & (eq_step >= equilibrate_steps)
+        if eq_finished.any():
+            pm = pts(eq_finished)
+            phi_guess[:, pm] = phi[:, pm]
+            phase[eq_finished] = first_phase
+            it[eq_finished], err_prev[eq_finished] = 0, np.inf
+        step_done = step_done & ~eq
+        step[step_done] += 1
+        n_steps += step_done
+        to_newton = step_done & (step >= bdf_maxsteps)
+        if to_newton.any():
+            pm = pts(to_newton)
+            phi_save[:, pm] = phi[:, pm]
+            phase[to_newton] = _NEWTON
+            it[to_newton], err_prev[to_newton] = 0, np.inf
+        if verbosity:
+            print(f"batch iteration: live {int(live.sum())}, "
+                  f"max |v| {np.max(delta[live]):.3e}")
+
+    # final temperature of each converged state
+    b.source(phi, vd, tg, pres, jac=False)
+    out = []
+    for j, (a, e) in enumerate(zip(b.offsets_np[:-1], b.offsets_np[1:])):
+        st = _array_to_state(phi[:, a:e].clone())
+        out.append(FlameletResult(st, b.T[a:e].clone(),
+                                  bool(phase[j] == _DONE_OK), int(n_newton[j]),
+                                  int(n_steps[j]), int(attempt[j]),
+                                  float(finished[j])))
+    return out

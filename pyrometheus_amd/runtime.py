@@ -153,12 +153,16 @@ MECH_SIGNATURES = {
     "pyro_creation_destruction": (_P, _P, _P, _I64, _P, _P, _I64, _I64, _P),
     "pyro_temperature": (_P, _I64, _P, _I64, _P, _I64, _P, _I64, _I, _I, _D,
                          _I, _P, _P, _P),
-    "pyro_wdot_jacobian": (_P, _P, _P, _I64, _P, _I64, _P, _I64, _I64, _P),
     "pyro_wdot_jvp": (_P, _P, _P, _I64, _P, _P, _P, _I64, _P, _P, _I64,
                       _I64, _P),
     "pyro_wdot_vjp": (_P, _P, _P, _I64, _P, _I64, _P, _P, _P, _I64, _I64, _P),
-    "pyro_flamelet_source": (_P, _I64, _P, _P, _D, _P, _I64, _P, _P, _I64,
-                             _I64, _D, _I, _P, _P),
+    "pyro_flamelet_source": (_P, _I64, _P, _P, _I64, _P, _I64, _P, _I64, _P,
+                             _P, _I64, _D, _I, _P, _P),
+    "pyro_flamelet_assemble": (_P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P,
+                               _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64,
+                               _P),
+    "pyro_flamelet_thomas": (_P, _P, _P, _P, _P, _P, _I64, _P, _P, _P),
+    "pyro_flamelet_update": (_P, _I64, _P, _P, _P, _P, _I64, _P, _I64, _P),
 }
 
 
