@@ -1,4 +1,5 @@
-"""Benchmark: net production rates of a 53-species FP64 mechanism.
+"""Benchmark: net production rates of a 53-species FP64 mechanism, and
+batched versus plain steady flamelet solves.
 
 One step = one ``get_net_production_rates`` evaluation (one fused HIP kernel)
 over 2^24 cells of GRI-3.0 (53 species, 325 reactions) with the state already
@@ -11,8 +12,18 @@ the kernel's average launch time, measured with HIP events on the stream it
 runs on), and the CPU baseline: the numpy oracle (the reference's op
 structure) timed on a bounded sample on this host.
 
+The ``flamelets`` object of the same line solves 256 steady flamelets of the
+9-species H2/air mechanism (grid sizes uniform in [64, 1024] clustered about
+Z_st, chi_st log-uniform in [1, 1000] 1/s, fuel H2:N2 at 300 K, air at 800 K,
+1 atm) with ``solve_batch`` (one launch sequence per iteration for every
+unfinished flamelet) and times the plain drop-in (one ``FlameletSolver`` per
+flamelet) on a sample of the same problems; both reach Newton |v| < 1e-8
+under the scaled update norm.  Latency of a batched flamelet is the time
+from submission of the batch to that flamelet's convergence.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N]
                        [--mech gri30] [--cpu-seconds S] [--no-cpu]
+                       [--flamelets B] [--plain-sample K] [--no-flamelets]
 """
 
 import argparse
@@ -27,6 +38,8 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
+METRIC = ("Mcells/s \u03c9\u0307 eval (53-sp FP64); flamelet solves/s + p99 "
+          "latency vs plain")
 
 
 def parse():
@@ -42,6 +55,13 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES "
                          "(3 launches); calibrates HBM byte counters")
+    ap.add_argument("--flamelets", type=int, default=256)
+    ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--flamelet-reps", type=int, default=3)
+    ap.add_argument("--flamelet-seed", type=int, default=4)
+    ap.add_argument("--no-flamelets", action="store_true")
+    ap.add_argument("--no-wdot", action="store_true",
+                    help="skip the omega-dot leg (profiling the flamelets)")
     ap.add_argument("--traffic-json",
                     default=os.path.join(ROOT, "profiles", "wdot_gri30_traffic.json"))
     return ap.parse_args()
@@ -96,6 +116,104 @@ def cpu_baseline(mech, seconds, seed):
                       f"{os.cpu_count()} host threads visible; JAX absent"}
 
 
+def flamelet_leg(args, device):
+    """Batched (``solve_batch``) against plain (``FlameletSolver``) solves of
+    the same flamelets; kernel times of the batched run from HIP events."""
+    import numpy as np
+    import torch
+    from pyrometheus_amd import bundled, get_thermochem_class
+    from pyrometheus_amd.flamelets.engine import FlameletBatch
+    from pyrometheus_amd.flamelets.solver import solve_batch
+    from pyrometheus_amd.flamelets.workload import (flamelet_batch,
+                                                    solve_plain)
+    mech = bundled("sandiego")
+    gas = get_thermochem_class(mech)(device=device)
+    doms, probs, chis = flamelet_batch(gas, n_flamelets=args.flamelets,
+                                       seed=args.flamelet_seed)
+    kw = dict(equilibrate_steps=24, newton_norm="scaled")
+    sizes = np.array([len(p.x) for p in probs])
+    print(f"[bench] flamelets: {len(probs)} batched, points {sizes.sum()}",
+          file=sys.stderr, flush=True)
+    solve_batch(gas, probs[:2], **kw)  # warm-up
+    torch.cuda.synchronize()
+    def batched(timed):
+        b = FlameletBatch(gas, [p.x for p in probs],
+                          np.stack([p.y_ox for p in probs]),
+                          np.stack([p.y_fu for p in probs]))
+        b.time_kernels(timed)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = solve_batch(gas, probs, batch=b, **kw)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, res, b
+
+    runs = []
+    for rep in range(args.flamelet_reps):
+        wall, res, _ = batched(False)
+        runs.append((wall, res))
+        print(f"[bench] batched rep {rep}: {wall:.3f} s, converged "
+              f"{sum(r.converged for r in res)}", file=sys.stderr, flush=True)
+    walls = [r[0] for r in runs]
+    wall, res = runs[int(np.argsort(walls)[len(walls) // 2])]
+    # one more run with HIP events around every kernel for the breakdown
+    _, _, b = batched(True)
+    kms = b.kernel_ms()
+    iters = len(b.events["pyro_flamelet_thomas"])
+    lat_b = np.array([r.finished_at for r in res])
+    plain_t, plain_ok = [], 0
+    for j in range(min(args.plain_sample, len(probs))):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, _, ok = solve_plain(gas, doms[j], probs[j], **kw)
+        torch.cuda.synchronize()
+        plain_t.append(time.perf_counter() - t0)
+        plain_ok += bool(ok)
+        print(f"[bench] plain {j}: N={len(probs[j].x)} {plain_t[-1]:.3f} s "
+              f"ok={ok}", file=sys.stderr, flush=True)
+    plain_t = np.array(plain_t)
+    batched_sps = len(probs) / wall
+    plain_sps = len(plain_t) / plain_t.sum()
+    nv = mech.n_species + 1
+    rows = int(sizes.sum()) * iters
+    thomas_ms = kms["pyro_flamelet_thomas"]
+    # block Thomas on (nv x nv) central blocks with scalar coupling: per
+    # block row it reads C (nv^2), rhs (nv), L and U, writes x (nv)
+    bytes_row = 8 * (nv * nv + 2 * nv + 2)
+    flop_row = 4.67 * nv ** 3 + 8 * nv ** 2
+    return {
+        "workload": f"{len(probs)} steady flamelets, {mech.name} "
+                    f"({mech.n_species} species), N in [64, 1024] "
+                    "Z-clustered, chi_st log-uniform [1, 1000] 1/s, "
+                    "T_ox 800 K, T_fu 300 K, 1 atm, equilibrium start",
+        "batched_solves_per_s": batched_sps,
+        "plain_solves_per_s": plain_sps,
+        "speedup_vs_plain": batched_sps / plain_sps,
+        "batched_wall_s": wall,
+        "batched_wall_s_reps": walls,
+        "batched_latency_p50_s": float(np.percentile(lat_b, 50)),
+        "batched_latency_p99_s": float(np.percentile(lat_b, 99)),
+        "plain_latency_p50_s": float(np.percentile(plain_t, 50)),
+        "plain_latency_p99_s": float(np.percentile(plain_t, 99)),
+        "plain_sample": f"first {len(plain_t)} of the same problems, one "
+                        "FlameletSolver each, sequential",
+        "converged_batched": f"{sum(r.converged for r in res)}/{len(res)}",
+        "converged_plain": f"{plain_ok}/{len(plain_t)}",
+        "newton_norm": "scaled (enthalpy row / max(|h_ox|, |h_fu|))",
+        "batch_iterations": iters,
+        "kernel_ms": kms,
+        "thomas_roofline": {
+            "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+            "achieved": bytes_row * rows / (thomas_ms * 1e-3) / 1e9,
+            "frac": bytes_row * rows / (thomas_ms * 1e-3) / 1e9
+            / HBM_PEAK_GBS,
+            "bytes_per_block_row": bytes_row,
+            "flop_per_block_row": flop_row,
+            "fp64_tflops": flop_row * rows / (thomas_ms * 1e-3) / 1e12,
+            "note": "one wavefront per flamelet, rows in sequence: "
+                    "latency-bound"},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -117,6 +235,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": src.numel() * 8}), flush=True)
+        return
+    if args.no_wdot:
+        line = {"metric": METRIC, "n_gpus": 1, "data": "synthetic",
+                "flamelets": flamelet_leg(args, device)}
+        print(json.dumps(line), flush=True)
         return
     mech = bundled(args.mech)
     gas = get_thermochem_class(mech)(device=device)
@@ -157,7 +280,7 @@ def main():
         if tj.get("mechanism") == mech.name and tj.get("cells") == n:
             traffic = tj.get("bytes_per_launch")
     line = {
-        "metric": "Mcells/s omega-dot eval (53-sp FP64)",
+        "metric": METRIC,
         "value": n / (ms_step * 1e-3) / 1e6,
         "unit": "Mcells/s",
         "n_gpus": 1,
@@ -188,6 +311,10 @@ def main():
                      "hbm_gbs_algorithmic": fl["bytes_per_cell"] * n
                      / (avg_launch * 1e-3) / 1e9},
     }
+    del rho, T, Y, out
+    torch.cuda.empty_cache()
+    if not args.no_flamelets:
+        line["flamelets"] = flamelet_leg(args, device)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(mech, args.cpu_seconds, args.seed)

@@ -290,23 +290,38 @@ k_flamelet_update(double* __restrict__ phi, int64_t ldp,
   for (int v = 0; v < NV; ++v) phi[v * ldp + i] += x[i * NV + v];
 }
 
-// Deterministic per-system sum of squares of x (P, NV): one block per
-// system, fixed-order strided partial sums and a fixed reduction tree.
+// Deterministic per-system sums of squares of x (P, NV), split into the
+// enthalpy row and the species rows: out[2 s] = sum v_h^2, out[2 s + 1] =
+// sum v_Y^2.  One block per system, fixed-order strided partial sums and a
+// fixed reduction tree, so a system's value does not depend on the batch.
 __global__ void __launch_bounds__(256)
 k_system_norm2(const double* __restrict__ x, const int64_t* __restrict__ off,
                double* __restrict__ out) {
-  __shared__ double part[256];
+  __shared__ double part[2][256];
   const int s = blockIdx.x;
   const int64_t b = off[s] * NV, e = off[s + 1] * NV;
-  double acc = 0.0;
-  for (int64_t j = b + threadIdx.x; j < e; j += 256) acc = fma(x[j], x[j], acc);
-  part[threadIdx.x] = acc;
+  double ah = 0.0, ay = 0.0;
+  for (int64_t j = b + threadIdx.x; j < e; j += 256) {
+    const double q = x[j] * x[j];
+    if ((j - b) % NV == 0)
+      ah += q;
+    else
+      ay += q;
+  }
+  part[0][threadIdx.x] = ah;
+  part[1][threadIdx.x] = ay;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    if (threadIdx.x < w) {
+      part[0][threadIdx.x] += part[0][threadIdx.x + w];
+      part[1][threadIdx.x] += part[1][threadIdx.x + w];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[s] = part[0];
+  if (threadIdx.x == 0) {
+    out[2 * s] = part[0][0];
+    out[2 * s + 1] = part[1][0];
+  }
 }
 
 }  // namespace pyro_fl

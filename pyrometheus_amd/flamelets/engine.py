@@ -14,8 +14,9 @@ launch sequence:
 3. ``pyro_flamelet_thomas`` -- block Thomas, one wave per flamelet, with the
    Laplacian coupling blocks as scalar multiples of the identity;
 4. ``pyro_flamelet_update`` -- phi += v for the selected flamelets and the
-   per-flamelet sum of squares of v (fixed-order reduction, so a flamelet's
-   result does not depend on what else is in the batch).
+   per-flamelet sums of squares of v over the enthalpy row and over the
+   species rows (fixed-order reductions, so a flamelet's result does not
+   depend on what else is in the batch).
 """
 
 from __future__ import annotations
@@ -77,7 +78,7 @@ class FlameletBatch:
         self.L = torch.empty(P, **dd)
         self.U = torch.empty(P, **dd)
         self.work = torch.empty(P * nv * (nv + 1), **dd)
-        self.norm2 = torch.empty(self.B, **dd)
+        self.norm2 = torch.empty((self.B, 2), **dd)
         self.T = torch.empty(P, **dd)
         self.n_fail = torch.zeros(1, dtype=torch.int32, device=self.dev)
         self.modes = torch.zeros(self.B, dtype=torch.int32, device=self.dev)
@@ -85,6 +86,30 @@ class FlameletBatch:
         self.active = torch.ones(self.B, dtype=torch.int32, device=self.dev)
         self.none_active = torch.zeros(self.B, dtype=torch.int32,
                                        device=self.dev)
+        # kernel name -> list of (start, end) HIP events when timing is on
+        self.events = None
+
+    def time_kernels(self, on=True):
+        """Record HIP events around every kernel launch (``kernel_ms``)."""
+        self.events = {} if on else None
+
+    def kernel_ms(self):
+        """Total milliseconds per kernel since ``time_kernels``."""
+        torch.cuda.synchronize(self.dev)
+        return {k: sum(a.elapsed_time(b) for a, b in v)
+                for k, v in (self.events or {}).items()}
+
+    def _call(self, name, *args):
+        if self.events is None:
+            return self.lib.call(name, *args)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        s = torch.cuda.current_stream(self.dev)
+        a.record(s)
+        st = self.lib.call(name, *args)
+        b.record(s)
+        self.events.setdefault(name, []).append((a, b))
+        return st
 
     # -- plumbing --------------------------------------------------------------
     def per_point(self, values) -> torch.Tensor:
@@ -108,7 +133,7 @@ class FlameletBatch:
     # -- kernels ---------------------------------------------------------------
     def source(self, phi, vd, tguess, p, jac: bool, tol=1e-10, maxit=100):
         self.n_fail.zero_()
-        self.lib.call("pyro_flamelet_source", phi.data_ptr(), phi.stride(0),
+        self._call("pyro_flamelet_source", phi.data_ptr(), phi.stride(0),
                       _ptr(vd), tguess.data_ptr(), 1, p.data_ptr(), 1,
                       self.S.data_ptr(), self.P, self.T.data_ptr(),
                       self.J.data_ptr() if jac else None, self.P, tol, maxit,
@@ -116,7 +141,7 @@ class FlameletBatch:
 
     def assemble(self, phi, chi, modes: Optional[torch.Tensor] = None,
                  phi_prev=None, r_prev=None, r_out=None):
-        self.lib.call("pyro_flamelet_assemble", phi.data_ptr(), phi.stride(0),
+        self._call("pyro_flamelet_assemble", phi.data_ptr(), phi.stride(0),
                       self.S.data_ptr(), self.P, self.J.data_ptr(),
                       chi.data_ptr(), self.la.data_ptr(), self.lb.data_ptr(),
                       self.lc.data_ptr(), self.sys.data_ptr(),
@@ -127,14 +152,14 @@ class FlameletBatch:
                       self.U.data_ptr(), _ptr(r_out), self.P, _stream())
 
     def thomas(self):
-        self.lib.call("pyro_flamelet_thomas", self.L.data_ptr(),
+        self._call("pyro_flamelet_thomas", self.L.data_ptr(),
                       self.J.data_ptr(), self.U.data_ptr(),
                       self.rhs.data_ptr(), self.x.data_ptr(),
                       self.off.data_ptr(), self.B, self.work.data_ptr(), None,
                       _stream())
 
     def update(self, phi, active: Optional[torch.Tensor]):
-        self.lib.call("pyro_flamelet_update", phi.data_ptr(), phi.stride(0),
+        self._call("pyro_flamelet_update", phi.data_ptr(), phi.stride(0),
                       self.x.data_ptr(), self.sys.data_ptr(), _ptr(active),
                       self.off.data_ptr(), self.B, self.norm2.data_ptr(),
                       self.P, _stream())
@@ -150,7 +175,8 @@ class FlameletBatch:
                       phi_prev=None, r_prev=None, apply=True, active=None):
         """One Newton update v = -J^-1 R for every flamelet (steady, or
         Crank-Nicolson where ``modes`` is 1).  Applies it when ``apply``;
-        returns the per-flamelet |v|^2 (device) and v (P, nv)."""
+        returns the per-flamelet sums of squares of v, (B, 2) on the device
+        as [enthalpy row, species rows], and v (P, nv)."""
         self.source(phi, vd, tguess, p, jac=True)
         self.assemble(phi, chi, modes, phi_prev, r_prev)
         self.thomas()

@@ -119,14 +119,47 @@ def _cpu(x):
     return x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else x
 
 
+NEWTON_NORMS = ("reference", "scaled")
+
+
+def enthalpy_scale(h_ox, h_fu):
+    """Magnitude of the enthalpy row, max(|h_ox|, |h_fu|, 1 J/kg): the scale
+    of its rounding error."""
+    return np.maximum(np.maximum(np.abs(h_ox), np.abs(h_fu)), 1.0)
+
+
+def update_norm(sums, kind="reference", h_scale=None):
+    """Newton update norm per flamelet from the sums of squares of v over
+    the enthalpy row and over the species rows (``sums`` (B, 2)).
+
+    ``"reference"``: |v|_2 over every row, the criterion of
+    ``solver.py:136-139``.  With h in J/kg its floor in double precision
+    rises steeply with N (the rounding error of the Laplacian of h grows
+    like N^2 |h|), so on fine grids it is never reached.  ``"scaled"``: the
+    enthalpy row is divided by ``h_scale`` (``enthalpy_scale``), the species
+    rows are kept as they are."""
+    sums = np.asarray(sums, dtype=float).reshape(-1, 2)
+    if kind == "reference":
+        return np.sqrt(sums[:, 0] + sums[:, 1])
+    if kind == "scaled":
+        return np.sqrt(sums[:, 0] / np.asarray(h_scale, dtype=float) ** 2
+                       + sums[:, 1])
+    raise ValueError(f"unknown Newton norm {kind!r}; use one of "
+                     f"{NEWTON_NORMS}")
+
+
 class FlameletSolver:
     """Plain drop-in: one flamelet, host loop per Newton iteration."""
 
-    def __init__(self, domain, pyro_gas, mass_frac_bcs):
+    def __init__(self, domain, pyro_gas, mass_frac_bcs,
+                 newton_norm="reference"):
         y_ox, y_fu = mass_frac_bcs
         self.domain = domain
         self.gov_eqns = FlameletEquations(pyro_gas, domain, y_ox, y_fu)
         self.num_v = pyro_gas.num_species + 1
+        if newton_norm not in NEWTON_NORMS:
+            raise ValueError(f"unknown Newton norm {newton_norm!r}")
+        self.newton_norm = newton_norm
         self.log = []
 
     # one Newton update v = -J^-1 R (solver.py:63-101)
@@ -148,7 +181,9 @@ class FlameletSolver:
         else:
             b.newton_update(phi, chi, vd, tg, p, apply=False)
         v = b.x.T.contiguous()
-        return _array_to_state(v), float(b.norm2[0].item()) ** 0.5
+        delta = update_norm(b.norm2.cpu().numpy(), self.newton_norm,
+                            enthalpy_scale(h_ox, h_fu))
+        return _array_to_state(v), float(delta[0])
 
     def flamelet_newton_step(self, state_guess, diss_rate, viscous_diss,
                              temp_guess, pressure, h_ox, h_fu):
@@ -318,12 +353,15 @@ def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
                 bdf_newton_tol=1e-9, bdf_time_step=1e-5, bdf_maxsteps=10,
                 try_newton=False, max_attempts=10, verbosity=False,
                 equilibrate_steps=0, equilibrate_dt0=1e-8,
-                equilibrate_growth=2.5,
+                equilibrate_growth=2.5, newton_norm="reference",
                 batch: Optional[FlameletBatch] = None) -> List[FlameletResult]:
     """Solve every problem with the control flow of ``FlameletSolver.solve``
     applied to each flamelet independently (preceded, when
     ``equilibrate_steps`` > 0, by ``FlameletSolver.equilibrate``); one launch
-    sequence per iteration advances every unfinished flamelet."""
+    sequence per iteration advances every unfinished flamelet.
+    ``newton_norm`` is the update norm of ``update_norm``."""
+    if newton_norm not in NEWTON_NORMS:
+        raise ValueError(f"unknown Newton norm {newton_norm!r}")
     t_start = time.time()
     B = len(problems)
     b = batch or FlameletBatch(pyro_gas, [p.x for p in problems],
@@ -339,8 +377,10 @@ def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
               for p in problems])
     tg = cat([p.t_guess for p in problems])
     pres = b.per_point([p.pressure for p in problems])
-    b.set_boundary_enthalpy(np.array([p.h_ox for p in problems]),
-                            np.array([p.h_fu for p in problems]))
+    h_ox = np.array([p.h_ox for p in problems], dtype=float)
+    h_fu = np.array([p.h_fu for p in problems], dtype=float)
+    b.set_boundary_enthalpy(h_ox, h_fu)
+    h_scale = enthalpy_scale(h_ox, h_fu)
     phi_prev = torch.empty_like(phi)
     r_prev = torch.empty_like(phi)
     r_tmp = torch.empty_like(phi)
@@ -393,7 +433,7 @@ def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
         # refresh the temperature guess of live flamelets only, so a finished
         # flamelet's final temperature does not depend on its neighbours
         tg.copy_(torch.where(pts(live), b.T, tg))
-        delta = np.sqrt(norm2.cpu().numpy())
+        delta = update_norm(norm2.cpu().numpy(), newton_norm, h_scale)
         now = time.time() - t_start
         newton = live & ((phase == _NEWTON_TRY) | (phase == _NEWTON))
         n_newton += newton

@@ -122,6 +122,26 @@ def core_lib_path() -> str:
     return os.path.join(GEN_DIR, f"libpyro_core_{d}.so")
 
 
+def prune_stale_libraries(keep) -> list:
+    """Delete generated sources and libraries under ``_gen/`` built from
+    older sources (their file names carry other digests) and not in
+    ``keep``; returns the removed paths."""
+    if not os.path.isdir(GEN_DIR):
+        return []
+    src = source_digest()
+    keep = {os.path.basename(k)[:-3] for k in keep}
+    removed = []
+    for f in sorted(os.listdir(GEN_DIR)):
+        stem, ext = os.path.splitext(f)
+        if ext not in (".so", ".hip") or not stem.startswith("libpyro_"):
+            continue
+        if stem in keep or stem.endswith("_" + src):
+            continue
+        os.remove(os.path.join(GEN_DIR, f))
+        removed.append(f)
+    return removed
+
+
 def build_core_library(force: bool = False) -> str:
     os.makedirs(GEN_DIR, exist_ok=True)
     so = core_lib_path()
