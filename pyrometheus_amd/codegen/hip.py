@@ -44,7 +44,8 @@ import math
 import os
 from typing import Dict, List, Optional, Sequence, Tuple
 
-from ..mechanism import GAS_CONSTANT, ONE_ATM, Mechanism, Reaction
+from ..mechanism import (GAS_CONSTANT, ONE_ATM, Mechanism, Reaction,
+                         as_mechanism)
 from . import CodeGenerationOptions, CodeGenerator
 
 ABI_VERSION = 1
@@ -813,7 +814,7 @@ class HipCodeGenerator(CodeGenerator):
                 "HIP code generation")
         if opts.scalar_type not in (None, "double"):
             raise TypeError("the hip target computes in double precision only")
-        mech = sol
+        mech = as_mechanism(sol)
         L = _Lowering(mech, HipCodeGenerator.tuning)
         w = _Writer()
         wdot_block = 256
@@ -848,5 +849,6 @@ class HipCodeGenerator(CodeGenerator):
         (``codegen/python.py:699-707``)."""
         from ..runtime import load_mechanism_library
         from ..thermochem import make_thermochem_class
-        lib = load_mechanism_library(sol)
-        return make_thermochem_class("Thermochemistry", lib, sol)
+        mech = as_mechanism(sol)
+        lib = load_mechanism_library(mech)
+        return make_thermochem_class("Thermochemistry", lib, mech)

@@ -565,3 +565,23 @@ def bundled(name: str) -> Mechanism:
         from .synthetic import synthetic_gri_like
         return synthetic_gri_like()
     return load_mechanism(os.path.join(MECH_DIR, f"{name}.yaml"), name=name)
+
+
+def as_mechanism(sol, phase: Optional[str] = None) -> Mechanism:
+    """The IR for what a code generator is handed: a ``Mechanism`` as is, a
+    path to a Cantera YAML file, or an object with Cantera's
+    ``Solution.write_yaml`` (a ``ct.Solution``, written to a temporary YAML
+    file and read back; Cantera itself is optional)."""
+    if isinstance(sol, Mechanism):
+        return sol
+    if isinstance(sol, (str, os.PathLike)):
+        return load_mechanism(os.fspath(sol), phase=phase)
+    if hasattr(sol, "write_yaml"):
+        import tempfile
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "mechanism.yaml")
+            sol.write_yaml(path)
+            return load_mechanism(path, phase=phase,
+                                  name=getattr(sol, "name", None)
+                                  or "mechanism")
+    raise TypeError(f"cannot read a mechanism from {type(sol).__name__}")

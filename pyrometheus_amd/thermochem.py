@@ -348,6 +348,9 @@ def make_thermochem_class(name: str, lib: HipLibrary, mech: Mechanism):
 
 
 def get_thermochem_class(mech: Mechanism):
-    """Class bound to ``mech``'s HIP library (built on first use if missing)."""
+    """Class bound to ``mech``'s HIP library (built on first use if missing).
+    ``mech`` may also be a Cantera YAML path or a ``ct.Solution``."""
+    from .mechanism import as_mechanism
+    mech = as_mechanism(mech)
     return make_thermochem_class("Thermochemistry",
                                  load_mechanism_library(mech), mech)

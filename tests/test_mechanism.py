@@ -126,3 +126,25 @@ def test_synthetic(name, counts):
             return (c[0] + c[1] / 2 * t + c[2] / 3 * t**2 + c[3] / 4 * t**3
                     + c[4] / 5 * t**4 + c[5] / t)
         assert math.isclose(h(s.low), h(s.high), rel_tol=1e-12, abs_tol=1e-9)
+
+
+def test_as_mechanism_paths_and_solution_like(tmp_path):
+    """A generator accepts the IR, a YAML path, or anything with Cantera's
+    ``write_yaml`` (stand-in object; Cantera is not installed)."""
+    import os
+    from pyrometheus_amd.mechanism import MECH_DIR, as_mechanism, bundled
+    m = bundled("sandiego")
+    assert as_mechanism(m) is m
+    p = os.path.join(MECH_DIR, "sandiego.yaml")
+    assert as_mechanism(p).digest() == m.digest()
+
+    class SolutionLike:
+        name = "sandiego"
+
+        def write_yaml(self, path):
+            with open(p) as src, open(path, "w") as dst:
+                dst.write(src.read())
+
+    assert as_mechanism(SolutionLike()).digest() == m.digest()
+    with pytest.raises(TypeError):
+        as_mechanism(3.0)
