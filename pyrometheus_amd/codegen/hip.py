@@ -326,17 +326,26 @@ class _Lowering:
         return (f"exp(fma({self.c(db)}, lnT, fma({self.c(-dta)}, invT, "
                 f"{self.c(dlna)})))")
 
-    def third_body(self, r: Reaction) -> str:
+    def third_body(self, r: Reaction, C: str = "C", Ctot: str = "Ctot") -> str:
+        """[M] = d Ctot + sum_k (eff_k - d) C_k (chem_expr.py:327-345); with
+        ``C``/``Ctot`` naming tangent arrays it gives the tangent of [M]."""
         d = r.default_efficiency
         terms = []
         for sp, eff in r.efficiencies:
             coef = eff - d
             if coef != 0.0:
                 terms.append((coef, self.idx[sp]))
-        expr = "0.0" if d == 0 else ("Ctot" if d == 1.0 else f"({self.c(d)} * Ctot)")
+        expr = "0.0" if d == 0 else (Ctot if d == 1.0 else f"({self.c(d)} * {Ctot})")
         for coef, k in terms:
-            expr = f"fma({self.c(coef)}, C[{k}], {expr})"
+            expr = f"fma({self.c(coef)}, {C}[{k}], {expr})"
         return expr
+
+    def efficiency_terms(self, r: Reaction) -> List[Tuple[int, float]]:
+        """(k, eff_k - d) for the explicit colliders of a three-body or
+        falloff reaction."""
+        d = r.default_efficiency
+        return [(self.idx[sp], e - d) for sp, e in r.efficiencies
+                if e - d != 0.0]
 
     def uses_ctot(self) -> bool:
         return any(r.default_efficiency != 0 for r in self.m.reactions
@@ -715,6 +724,151 @@ def _emit_wdot_jac(w: _Writer, L: _Lowering):
     w("")
 
 
+def _reverse_parts(w: _Writer, L: _Lowering, r: Reaction):
+    """Emit pf, pb, kinv, dkinvT (reversible) and the partial-derivative
+    maps of the forward and reverse products; returns (dpf, dpb, cols)."""
+    rpairs = L.reactant_pairs(r)
+    w(f"const double pf = {L.prod_expr(rpairs)};")
+    dpf = dict(L.prod_partials(rpairs))
+    dpb = {}
+    if r.reversible:
+        ppairs = [(L.idx[sp], nu) for sp, nu in r.products]
+        w(f"const double pb = {L.prod_expr(ppairs)};")
+        dpb = dict(L.prod_partials(ppairs))
+        kinv = L.kinv_expr(r) if L.species_kc else f"exp({L.nlk_expr(r)})"
+        w(f"const double kinv = {kinv};")
+        dnu = sum(nu for _, nu in r.products) - sum(nu for _, nu in r.reactants)
+        hs = [f"{lit(-nu)} * hrt[{L.idx[sp]}]" for sp, nu in r.products]
+        hs += [f"{lit(nu)} * hrt[{L.idx[sp]}]" for sp, nu in r.reactants]
+        w(f"const double dkinvT = kinv * invT * ({lit(dnu)} + "
+          + " + ".join(hs) + ");")
+        w("const double delta = fma(-kinv, pb, pf);")
+    else:
+        w("const double delta = pf;")
+    return dpf, dpb, sorted(set(dpf) | set(dpb))
+
+
+def _deriv_prologue(w: _Writer, L: _Lowering):
+    rev = any(r.reversible for r in L.m.reactions)
+    if L.species_kc:
+        _kin_prologue(w, L, need_g=False)
+        if rev:
+            L.emit_gibbs_exp(w)
+    else:
+        _kin_prologue(w, L, need_g=rev)
+    if rev:
+        w("double hrt[NS];")
+        w("species_h_rt(T, invT, hrt);")
+
+
+def _emit_wdot_jvp(w: _Writer, L: _Lowering):
+    """Forward mode, reaction-major: w and dw = dw/dT dT + dw/dC dC in one
+    pass, one cell per lane, no Jacobian held."""
+    m = L.m
+    w("// omega-dot and its directional derivative (forward mode)")
+    w("__device__ __forceinline__ void net_production_rates_jvp("
+      "const double T, const double (&C)[NS], const double dT, "
+      "const double (&dC)[NS], double (&w)[NS], double (&dw)[NS]) {")
+    w.ind += 1
+    _deriv_prologue(w, L)
+    if L.uses_ctot():
+        w("double dCtot = 0.0;")
+        w("#pragma unroll")
+        w("for (int k = 0; k < NS; ++k) dCtot += dC[k];")
+    w("#pragma unroll")
+    w("for (int k = 0; k < NS; ++k) { w[k] = 0.0; dw[k] = 0.0; }")
+    pending = set()
+    for i, r in enumerate(m.reactions):
+        w(f"{{  // R{i}: {r.equation} [{r.reaction_type}]")
+        w.ind += 1
+        L.kf_block_deriv(w, r)
+        dpf, dpb, cols = _reverse_parts(w, L, r)
+        fv = " + ".join(f"({e}) * dC[{j}]" for j, e in dpf.items()) or "0.0"
+        if r.reversible:
+            bv = " + ".join(f"({e}) * dC[{j}]" for j, e in dpb.items()) or "0.0"
+            w(f"const double ddelta = fma(-kinv, {bv}, {fv}) - dkinvT * dT * pb;")
+        else:
+            w(f"const double ddelta = {fv};")
+        dk = "dkfT * dT"
+        if r.kind in ("three-body", "falloff"):
+            w(f"const double dM = {L.third_body(r, 'dC', 'dCtot')};")
+            dk = "fma(dkfM, dM, dkfT * dT)"
+        w("const double q = kf * delta;")
+        w(f"const double dq = fma({dk}, delta, kf * ddelta);")
+        upd = L.species_updates(r, "q")
+        for k, nu in upd:
+            nl = lit(nu)
+            w(f"w[{k}] = fma({nl}, q, w[{k}]);")
+            w(f"dw[{k}] = fma({nl}, dq, dw[{k}]);")
+        pending.update(k for k, _ in upd)
+        if (i + 1) % L.tuning.get("pin_every", 1) == 0 or i + 1 == len(m.reactions):
+            L.pin(w, [f"w[{k}]" for k in sorted(pending)]
+                  + [f"dw[{k}]" for k in sorted(pending)])
+            pending.clear()
+        w.ind -= 1
+        w("}")
+    w.ind -= 1
+    w("}")
+    w("")
+
+
+def _emit_wdot_vjp(w: _Writer, L: _Lowering):
+    """Reverse mode, reaction-major: given the cotangent wb of w, accumulate
+    Cb = wb^T dw/dC and Tb = wb^T dw/dT in one pass."""
+    m = L.m
+    w("// cotangent of omega-dot pulled back to (C, T) (reverse mode)")
+    w("__device__ __forceinline__ void net_production_rates_vjp("
+      "const double T, const double (&C)[NS], const double (&wb)[NS], "
+      "double (&Cb)[NS], double& Tb) {")
+    w.ind += 1
+    _deriv_prologue(w, L)
+    w("#pragma unroll")
+    w("for (int k = 0; k < NS; ++k) Cb[k] = 0.0;")
+    w("double tb = 0.0, mbd = 0.0;  // T cotangent; default-efficiency sum")
+    pending = set()
+    for i, r in enumerate(m.reactions):
+        w(f"{{  // R{i}: {r.equation} [{r.reaction_type}]")
+        w.ind += 1
+        upd = L.species_updates(r, "q")
+        rb = " + ".join(f"{lit(nu)} * wb[{k}]" for k, nu in upd) or "0.0"
+        w(f"const double rb = {rb};")
+        L.kf_block_deriv(w, r)
+        dpf, dpb, cols = _reverse_parts(w, L, r)
+        if r.reversible:
+            w("tb = fma(rb, fma(dkfT, delta, -kf * dkinvT * pb), tb);")
+        else:
+            w("tb = fma(rb, dkfT * delta, tb);")
+        w("const double g = rb * kf;")
+        for j in cols:
+            a, b = dpf.get(j), dpb.get(j)
+            if a is not None and b is not None:
+                w(f"Cb[{j}] = fma(g, fma(-kinv, {b}, {a}), Cb[{j}]);")
+            elif a is not None:
+                w(f"Cb[{j}] = fma(g, {a}, Cb[{j}]);")
+            else:
+                w(f"Cb[{j}] = fma(-g * kinv, {b}, Cb[{j}]);")
+        touched = set(cols)
+        if r.kind in ("three-body", "falloff"):
+            w("const double mb = rb * dkfM * delta;")
+            for j, e in L.efficiency_terms(r):
+                w(f"Cb[{j}] = fma({L.c(e)}, mb, Cb[{j}]);")
+                touched.add(j)
+            if r.default_efficiency != 0.0:
+                w(f"mbd = fma({L.c(r.default_efficiency)}, mb, mbd);")
+        pending.update(touched)
+        if (i + 1) % L.tuning.get("pin_every", 1) == 0 or i + 1 == len(m.reactions):
+            L.pin(w, [f"Cb[{k}]" for k in sorted(pending)] + ["tb", "mbd"])
+            pending.clear()
+        w.ind -= 1
+        w("}")
+    w("#pragma unroll")
+    w("for (int k = 0; k < NS; ++k) Cb[k] += mbd;")
+    w("Tb = tb;")
+    w.ind -= 1
+    w("}")
+    w("")
+
+
 def _emit_reaction_quantities(w: _Writer, L: _Lowering):
     m = L.m
     w("// per-reaction k_f, -ln K_c, r_f, r_r, r_net to memory "
@@ -823,6 +977,8 @@ class HipCodeGenerator(CodeGenerator):
         _emit_wdot(w, L)
         _emit_reaction_quantities(w, L)
         _emit_creation_destruction(w, L)
+        _emit_wdot_jvp(w, L)
+        _emit_wdot_vjp(w, L)
         dense = mech.n_species <= DENSE_JAC_MAX_SPECIES
         if dense:
             _emit_wdot_jac(w, L)

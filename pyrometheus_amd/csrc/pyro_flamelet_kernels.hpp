@@ -142,74 +142,6 @@ k_flamelet_source(const double* __restrict__ phi, int64_t ldp,
   if (!ok && n_fail) atomicAdd(n_fail, 1);
 }
 
-// JVP / VJP of w(rho, T, Y) through the dense kinetics Jacobian.
-//   dw_k = sum_j J_kj (Y_j v_rho + rho v_Yj)/W_j + JT_k v_T
-//   g_rho = sum_kj wb_k J_kj Y_j/W_j, g_T = sum_k wb_k JT_k,
-//   g_Yj = rho/W_j sum_k wb_k J_kj
-__global__ void __launch_bounds__(256)
-k_wdot_jvp(const double* __restrict__ rho, const double* __restrict__ T,
-           const double* __restrict__ Y, int64_t ldY,
-           const double* __restrict__ vr, const double* __restrict__ vT,
-           const double* __restrict__ vY, int64_t ldV,
-           double* __restrict__ wout, double* __restrict__ dw, int64_t ldO,
-           int64_t n) {
-  const int64_t i = cell_index();
-  if (i >= n) return;
-  const double r = rho[i];
-  double C[NS], y[NS], w[NS], J[NS][NS], JT[NS];
-#pragma unroll
-  for (int k = 0; k < NS; ++k) {
-    y[k] = Y[k * ldY + i];
-    C[k] = mech::kInvW[k] * r * y[k];
-  }
-  mech::net_production_rates_jac(T[i], C, w, J, JT);
-  const double a = vr ? vr[i] : 0.0, t = vT ? vT[i] : 0.0;
-  double dC[NS];
-#pragma unroll
-  for (int j = 0; j < NS; ++j)
-    dC[j] = mech::kInvW[j] * fma(y[j], a, vY ? r * vY[j * ldV + i] : 0.0);
-#pragma unroll
-  for (int k = 0; k < NS; ++k) {
-    double acc = JT[k] * t;
-#pragma unroll
-    for (int j = 0; j < NS; ++j) acc = fma(J[k][j], dC[j], acc);
-    dw[k * ldO + i] = acc;
-    if (wout) wout[k * ldO + i] = w[k];
-  }
-}
-
-__global__ void __launch_bounds__(256)
-k_wdot_vjp(const double* __restrict__ rho, const double* __restrict__ T,
-           const double* __restrict__ Y, int64_t ldY,
-           const double* __restrict__ wb, int64_t ldB,
-           double* __restrict__ g_rho, double* __restrict__ g_T,
-           double* __restrict__ g_Y, int64_t ldG, int64_t n) {
-  const int64_t i = cell_index();
-  if (i >= n) return;
-  const double r = rho[i];
-  double C[NS], y[NS], w[NS], J[NS][NS], JT[NS], b[NS];
-#pragma unroll
-  for (int k = 0; k < NS; ++k) {
-    y[k] = Y[k * ldY + i];
-    C[k] = mech::kInvW[k] * r * y[k];
-    b[k] = wb[k * ldB + i];
-  }
-  mech::net_production_rates_jac(T[i], C, w, J, JT);
-  double gt = 0.0, gr = 0.0;
-#pragma unroll
-  for (int k = 0; k < NS; ++k) gt = fma(b[k], JT[k], gt);
-#pragma unroll
-  for (int j = 0; j < NS; ++j) {
-    double cj = 0.0;  // sum_k b_k J_kj = d<b,w>/dC_j
-#pragma unroll
-    for (int k = 0; k < NS; ++k) cj = fma(b[k], J[k][j], cj);
-    gr = fma(cj, y[j] * mech::kInvW[j], gr);
-    if (g_Y) g_Y[j * ldG + i] = cj * r * mech::kInvW[j];
-  }
-  if (g_rho) g_rho[i] = gr;
-  if (g_T) g_T[i] = gt;
-}
-
 // Residual and block Jacobian of the steady (mode 0) or Crank-Nicolson
 // (mode 1) flamelet equations for a batch of flamelets whose points are
 // concatenated; point i belongs to system sys[i] = s with rows
@@ -342,33 +274,6 @@ PYRO_API int pyro_flamelet_source(const double* phi, int64_t ldp,
   pyro_fl::k_flamelet_source<<<pyro::grid_for(n, 256), 256, 0, stream>>>(
       phi, ldp, vd, tguess, tg_stride, p, p_stride, S, lds, T_out, J, n, tol,
       maxit, n_fail);
-  return pyro::launch_status();
-}
-
-PYRO_API int pyro_wdot_jvp(const double* rho, const double* T, const double* Y,
-                           int64_t ldY, const double* v_rho, const double* v_T,
-                           const double* v_Y, int64_t ldV, double* wdot,
-                           double* jvp, int64_t ldO, int64_t n,
-                           hipStream_t stream) {
-  if (n == 0) return pyro::kOk;
-  if (n < 0 || ldY < n || ldO < n || (v_Y && ldV < n) || !rho || !T || !Y ||
-      !jvp)
-    return pyro::kInvalidArgument;
-  pyro_fl::k_wdot_jvp<<<pyro::grid_for(n, 256), 256, 0, stream>>>(
-      rho, T, Y, ldY, v_rho, v_T, v_Y, ldV, wdot, jvp, ldO, n);
-  return pyro::launch_status();
-}
-
-PYRO_API int pyro_wdot_vjp(const double* rho, const double* T, const double* Y,
-                           int64_t ldY, const double* wbar, int64_t ldB,
-                           double* g_rho, double* g_T, double* g_Y,
-                           int64_t ldG, int64_t n, hipStream_t stream) {
-  if (n == 0) return pyro::kOk;
-  if (n < 0 || ldY < n || ldB < n || (g_Y && ldG < n) || !rho || !T || !Y ||
-      !wbar)
-    return pyro::kInvalidArgument;
-  pyro_fl::k_wdot_vjp<<<pyro::grid_for(n, 256), 256, 0, stream>>>(
-      rho, T, Y, ldY, wbar, ldB, g_rho, g_T, g_Y, ldG, n);
   return pyro::launch_status();
 }
 

@@ -223,6 +223,149 @@ k_creation_destruction(const double* __restrict__ rho,
 }
 
 // ---------------------------------------------------------------------------
+// derivatives of w(rho, T, Y) with C_k = rho Y_k / W_k
+// ---------------------------------------------------------------------------
+// JVP: dC_k = (Y_k v_rho + rho v_Yk) / W_k, one forward-mode pass.
+__global__ void __launch_bounds__(kBlock)
+k_wdot_jvp(const double* __restrict__ rho, const double* __restrict__ T,
+           const double* __restrict__ Y, int64_t ldY,
+           const double* __restrict__ vr, const double* __restrict__ vT,
+           const double* __restrict__ vY, int64_t ldV,
+           double* __restrict__ wout, double* __restrict__ dwout, int64_t ldO,
+           int64_t n) {
+  const int64_t i = cell_index();
+  if (i >= n) return;
+  const double r = rho[i];
+  const double a = vr ? vr[i] : 0.0;
+  double C[mech::NS], dC[mech::NS], w[mech::NS], dw[mech::NS];
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k) {
+    const double y = Y[k * ldY + i];
+    C[k] = mech::kInvW[k] * r * y;
+    dC[k] = mech::kInvW[k] * fma(y, a, vY ? r * vY[k * ldV + i] : 0.0);
+  }
+  mech::net_production_rates_jvp(T[i], C, vT ? vT[i] : 0.0, dC, w, dw);
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k) {
+    dwout[k * ldO + i] = dw[k];
+    if (wout) wout[k * ldO + i] = w[k];
+  }
+}
+
+// VJP: Cb = wb^T dw/dC, Tb = wb^T dw/dT in one reverse pass, then
+// g_rho = sum_k Cb_k Y_k / W_k, g_Yk = Cb_k rho / W_k, g_T = Tb.
+__global__ void __launch_bounds__(kBlock)
+k_wdot_vjp(const double* __restrict__ rho, const double* __restrict__ T,
+           const double* __restrict__ Y, int64_t ldY,
+           const double* __restrict__ wb, int64_t ldB,
+           double* __restrict__ g_rho, double* __restrict__ g_T,
+           double* __restrict__ g_Y, int64_t ldG, int64_t n) {
+  const int64_t i = cell_index();
+  if (i >= n) return;
+  const double r = rho[i];
+  double C[mech::NS], b[mech::NS], Cb[mech::NS], Tb;
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k) {
+    C[k] = mech::kInvW[k] * r * Y[k * ldY + i];
+    b[k] = wb[k * ldB + i];
+  }
+  mech::net_production_rates_vjp(T[i], C, b, Cb, Tb);
+  double gr = 0.0;
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k) {
+    const double y = Y[k * ldY + i];
+    gr = fma(Cb[k] * mech::kInvW[k], y, gr);
+    if (g_Y) g_Y[k * ldG + i] = Cb[k] * r * mech::kInvW[k];
+  }
+  if (g_rho) g_rho[i] = gr;
+  if (g_T) g_T[i] = Tb;
+}
+
+// Jacobian of the flamelet source S = [vd/rho; W_k w_k/rho] with respect to
+// phi = [h; Y] at a state whose temperature T is consistent with h, and
+// rho = p W/(R T) (flamelets/equations.py:361-389 differentiated as in
+// :124-129).  Column c is one forward-mode pass along
+//   c = 0:      dT = 1/cp,              drho = -rho/T dT
+//   c = 1 + m:  dT = -h_m/cp,           drho = -rho W/W_m - rho/T dT
+// with dC_j = (Y_j/W_j) drho + delta_jm rho/W_m.  Entry (r, c) of cell i
+// goes to out[(r*NV + c)*s_rc + i*s_i]; S itself to S[v*ldS + i] if given.
+__global__ void __launch_bounds__(kBlock)
+k_source_jacobian(const double* __restrict__ T, const double* __restrict__ p,
+                  int64_t p_stride, const double* __restrict__ Y, int64_t ldY,
+                  const double* __restrict__ vd, double* __restrict__ out,
+                  int64_t s_rc, int64_t s_i, double* __restrict__ S,
+                  int64_t ldS, int64_t n) {
+  constexpr int NS = mech::NS, NV = NS + 1;
+  const int64_t i = cell_index();
+  if (i >= n) return;
+  const double t = T[i];
+  double y[NS];
+  double sy = 0.0;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    y[k] = Y[k * ldY + i];
+    sy = fma(y[k], mech::kInvW[k], sy);
+  }
+  const double mmw = 1.0 / sy;
+  const double rho = p[i * p_stride] * mmw / (kGasConstant * t);
+  const double irho = 1.0 / rho;
+  const double v = vd ? vd[i] : 0.0;
+  double C[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) C[k] = mech::kInvW[k] * rho * y[k];
+  double cp;
+  {
+    double cpr[NS];
+    mech::species_cp_r(t, cpr);
+    double sc = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) sc = fma(y[k] * mech::kInvW[k], cpr[k], sc);
+    cp = kGasConstant * sc;
+  }
+  const double rT = rho / t;
+  double* o = out + i * s_i;
+#pragma unroll 1
+  for (int c = 0; c < NV; ++c) {
+    double dT, dr;
+    int m = c - 1;
+    if (c == 0) {
+      dT = 1.0 / cp;
+      dr = -rT * dT;
+    } else {
+      double hrt[NS];
+      mech::species_h_rt(t, 1.0 / t, hrt);
+      double hm = 0.0;
+#pragma unroll
+      for (int k = 0; k < NS; ++k)
+        if (k == m) hm = hrt[k] * mech::kInvW[k];
+      dT = -kGasConstant * t * hm / cp;
+      double iw = 0.0;
+#pragma unroll
+      for (int k = 0; k < NS; ++k)
+        if (k == m) iw = mech::kInvW[k];
+      dr = fma(-rT, dT, -rho * mmw * iw);
+    }
+    double dC[NS], w[NS], dw[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+      dC[k] = fma(y[k] * mech::kInvW[k], dr,
+                  k == m ? rho * mech::kInvW[k] : 0.0);
+    mech::net_production_rates_jvp(t, C, dT, dC, w, dw);
+    o[(0 * NV + c) * s_rc] = -v * irho * irho * dr;
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+      o[((1 + k) * NV + c) * s_rc] =
+          mech::kW[k] * irho * fma(-w[k] * irho, dr, dw[k]);
+    if (S && c == 0) {
+      S[i] = v * irho;
+#pragma unroll
+      for (int k = 0; k < NS; ++k)
+        S[(1 + k) * ldS + i] = mech::kW[k] * w[k] * irho;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // temperature: per-lane Newton on e(T,Y)=e* (do_energy) or h(T,Y)=h*
 // (codegen/python.py:375-396; flamelets/make_pyro.py:255-299).
 // mode 0: step, then stop when |step| < tol (generated get_temperature)
@@ -402,5 +545,50 @@ PYRO_API int pyro_temperature(const double* target, int64_t target_stride,
   k_temperature<<<pyro::grid_for(n, kBlock), kBlock, 0, stream>>>(
       target, target_stride, t_guess, guess_stride, Y, ldY, T_out, n,
       do_energy, mode, tol, maxit, n_fail, iters);
+  return pyro::launch_status();
+}
+
+PYRO_API int pyro_wdot_jvp(const double* rho, const double* T, const double* Y,
+                           int64_t ldY, const double* v_rho, const double* v_T,
+                           const double* v_Y, int64_t ldV, double* wdot,
+                           double* jvp, int64_t ldO, int64_t n,
+                           hipStream_t stream) {
+  using namespace pyro_kern;
+  if (n == 0) return pyro::kOk;
+  if (n < 0 || ldY < n || ldO < n || (v_Y && ldV < n) || !rho || !T || !Y ||
+      !jvp)
+    return pyro::kInvalidArgument;
+  k_wdot_jvp<<<pyro::grid_for(n, kBlock), kBlock, 0, stream>>>(
+      rho, T, Y, ldY, v_rho, v_T, v_Y, ldV, wdot, jvp, ldO, n);
+  return pyro::launch_status();
+}
+
+PYRO_API int pyro_wdot_vjp(const double* rho, const double* T, const double* Y,
+                           int64_t ldY, const double* wbar, int64_t ldB,
+                           double* g_rho, double* g_T, double* g_Y,
+                           int64_t ldG, int64_t n, hipStream_t stream) {
+  using namespace pyro_kern;
+  if (n == 0) return pyro::kOk;
+  if (n < 0 || ldY < n || ldB < n || (g_Y && ldG < n) || !rho || !T || !Y ||
+      !wbar)
+    return pyro::kInvalidArgument;
+  k_wdot_vjp<<<pyro::grid_for(n, kBlock), kBlock, 0, stream>>>(
+      rho, T, Y, ldY, wbar, ldB, g_rho, g_T, g_Y, ldG, n);
+  return pyro::launch_status();
+}
+
+PYRO_API int pyro_source_jacobian(const double* T, const double* p,
+                                  int64_t p_stride, const double* Y,
+                                  int64_t ldY, const double* vd, double* out,
+                                  int64_t s_rc, int64_t s_i, double* S,
+                                  int64_t ldS, int64_t n,
+                                  hipStream_t stream) {
+  using namespace pyro_kern;
+  if (n == 0) return pyro::kOk;
+  if (n < 0 || ldY < n || !T || !p || !Y || !out || s_rc <= 0 || s_i <= 0 ||
+      (S && ldS < n))
+    return pyro::kInvalidArgument;
+  k_source_jacobian<<<pyro::grid_for(n, kBlock), kBlock, 0, stream>>>(
+      T, p, p_stride, Y, ldY, vd, out, s_rc, s_i, S, ldS, n);
   return pyro::launch_status();
 }

@@ -38,6 +38,17 @@ def _ptr(t):
     return None if t is None else t.data_ptr()
 
 
+def _rows(t, name, writable=False):
+    """A contiguous (rows, points) array, the layout the kernels read with
+    one leading dimension for all state arrays; read-only inputs in another
+    layout are copied, arrays the kernels write must already conform."""
+    if t is None or t.is_contiguous():
+        return t
+    if writable:
+        raise ValueError(f"{name} must be a contiguous (rows, points) array")
+    return t.contiguous()
+
+
 class FlameletBatch:
     def __init__(self, gas, grids: Sequence[np.ndarray], y_ox, y_fu,
                  device=None):
@@ -132,6 +143,8 @@ class FlameletBatch:
 
     # -- kernels ---------------------------------------------------------------
     def source(self, phi, vd, tguess, p, jac: bool, tol=1e-10, maxit=100):
+        phi = _rows(phi, "phi")
+        vd, tguess, p = (_rows(x, "input") for x in (vd, tguess, p))
         self.n_fail.zero_()
         self._call("pyro_flamelet_source", phi.data_ptr(), phi.stride(0),
                       _ptr(vd), tguess.data_ptr(), 1, p.data_ptr(), 1,
@@ -141,6 +154,9 @@ class FlameletBatch:
 
     def assemble(self, phi, chi, modes: Optional[torch.Tensor] = None,
                  phi_prev=None, r_prev=None, r_out=None):
+        phi, chi, phi_prev, r_prev = (_rows(x, "input") for x in
+                                      (phi, chi, phi_prev, r_prev))
+        r_out = _rows(r_out, "r_out", writable=True)
         self._call("pyro_flamelet_assemble", phi.data_ptr(), phi.stride(0),
                       self.S.data_ptr(), self.P, self.J.data_ptr(),
                       chi.data_ptr(), self.la.data_ptr(), self.lb.data_ptr(),
@@ -159,6 +175,7 @@ class FlameletBatch:
                       _stream())
 
     def update(self, phi, active: Optional[torch.Tensor]):
+        phi = _rows(phi, "phi", writable=True)
         self._call("pyro_flamelet_update", phi.data_ptr(), phi.stride(0),
                       self.x.data_ptr(), self.sys.data_ptr(), _ptr(active),
                       self.off.data_ptr(), self.B, self.norm2.data_ptr(),

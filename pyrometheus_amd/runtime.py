@@ -176,6 +176,8 @@ MECH_SIGNATURES = {
     "pyro_wdot_jvp": (_P, _P, _P, _I64, _P, _P, _P, _I64, _P, _P, _I64,
                       _I64, _P),
     "pyro_wdot_vjp": (_P, _P, _P, _I64, _P, _I64, _P, _P, _P, _I64, _I64, _P),
+    "pyro_source_jacobian": (_P, _P, _I64, _P, _I64, _P, _P, _I64, _I64, _P,
+                             _I64, _I64, _P),
     "pyro_flamelet_source": (_P, _I64, _P, _P, _I64, _P, _I64, _P, _I64, _P,
                              _P, _I64, _D, _I, _P, _P),
     "pyro_flamelet_assemble": (_P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P,
