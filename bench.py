@@ -60,6 +60,9 @@ def parse():
     ap.add_argument("--flamelet-reps", type=int, default=3)
     ap.add_argument("--flamelet-seed", type=int, default=4)
     ap.add_argument("--no-flamelets", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=16.0)
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-wdot", action="store_true",
                     help="skip the omega-dot leg (profiling the flamelets)")
     ap.add_argument("--traffic-json",
@@ -122,7 +125,6 @@ def flamelet_leg(args, device):
     import numpy as np
     import torch
     from pyrometheus_amd import bundled, get_thermochem_class
-    from pyrometheus_amd.flamelets.engine import FlameletBatch
     from pyrometheus_amd.flamelets.solver import solve_batch
     from pyrometheus_amd.flamelets.workload import (flamelet_batch,
                                                     solve_plain)
@@ -136,29 +138,28 @@ def flamelet_leg(args, device):
           file=sys.stderr, flush=True)
     solve_batch(gas, probs[:2], **kw)  # warm-up
     torch.cuda.synchronize()
-    def batched(timed):
-        b = FlameletBatch(gas, [p.x for p in probs],
-                          np.stack([p.y_ox for p in probs]),
-                          np.stack([p.y_fu for p in probs]))
-        b.time_kernels(timed)
+    def batched(events=None):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        res = solve_batch(gas, probs, batch=b, **kw)
+        res = solve_batch(gas, probs, kernel_events=events, **kw)
         torch.cuda.synchronize()
-        return time.perf_counter() - t0, res, b
+        return time.perf_counter() - t0, res
 
     runs = []
     for rep in range(args.flamelet_reps):
-        wall, res, _ = batched(False)
+        wall, res = batched()
         runs.append((wall, res))
         print(f"[bench] batched rep {rep}: {wall:.3f} s, converged "
               f"{sum(r.converged for r in res)}", file=sys.stderr, flush=True)
     walls = [r[0] for r in runs]
     wall, res = runs[int(np.argsort(walls)[len(walls) // 2])]
     # one more run with HIP events around every kernel for the breakdown
-    _, _, b = batched(True)
-    kms = b.kernel_ms()
-    iters = len(b.events["pyro_flamelet_thomas"])
+    events, stats = {}, {}
+    torch.cuda.synchronize()
+    solve_batch(gas, probs, kernel_events=events, stats=stats, **kw)
+    torch.cuda.synchronize()
+    kms = {k: sum(a.elapsed_time(e) for a, e in v) for k, v in events.items()}
+    iters = len(events["pyro_flamelet_thomas"])
     lat_b = np.array([r.finished_at for r in res])
     plain_t, plain_ok = [], 0
     for j in range(min(args.plain_sample, len(probs))):
@@ -174,7 +175,7 @@ def flamelet_leg(args, device):
     batched_sps = len(probs) / wall
     plain_sps = len(plain_t) / plain_t.sum()
     nv = mech.n_species + 1
-    rows = int(sizes.sum()) * iters
+    rows = stats["point_iterations"]  # block rows through block Thomas
     thomas_ms = kms["pyro_flamelet_thomas"]
     # block Thomas on (nv x nv) central blocks with scalar coupling: per
     # block row it reads C (nv^2), rhs (nv), L and U, writes x (nv)
@@ -214,6 +215,55 @@ def flamelet_leg(args, device):
     }
 
 
+def serving_leg(args, device):
+    """Poisson arrivals of production-rate requests (2^10..2^20 cells,
+    9- and 53-species mechanisms) and flamelet solves (64..2048 points,
+    9-species), served by the continuous-batching server and by the plain
+    drop-in one request at a time (FIFO), on the same arrival trace."""
+    import numpy as np
+    import torch
+    from pyrometheus_amd import bundled, get_thermochem_class
+    from pyrometheus_amd.serving import (ChemistryServer, poisson_workload,
+                                         serve_plain, summarize)
+    gases = {n: get_thermochem_class(bundled(n))(device=device)
+             for n in ("sandiego", "gri30")}
+    arrivals = poisson_workload(gases, args.serve_requests, args.serve_rate,
+                                seed=5)
+    warm = poisson_workload(gases, 4, 100.0, seed=6, wdot_log2_cells=(10, 12),
+                            flamelet_points=(64, 96))
+    ChemistryServer(gases).run(warm)
+    print(f"[bench] serving: {len(arrivals)} requests at "
+          f"{args.serve_rate}/s", file=sys.stderr, flush=True)
+    srv = ChemistryServer(gases)
+    tickets = srv.run(arrivals)
+    lat = np.array([t.latency for t in tickets])
+    t_first = min(t.t_arrival for t in tickets)
+    t_last = max(t.t_done for t in tickets)
+    kinds = np.array([t.kind for t in tickets])
+    out = {"workload": f"{len(arrivals)} Poisson arrivals at "
+                       f"{args.serve_rate}/s (seed 5): 50% omega-dot of "
+                       "2^U{10..20} cells on sandiego or gri30, 50% flamelet "
+                       "solves of U{64..2048} points on sandiego",
+           "batched": summarize(lat, t_first, t_last),
+           "batched_by_kind": {k: summarize(lat[kinds == k], t_first, t_last)
+                               for k in ("wdot", "flamelet")},
+           "peak_hbm_bytes": srv.peak_used,
+           "flamelets_converged": int(sum(
+               t.result.converged for t in tickets if t.kind == "flamelet"))}
+    print("[bench] serving: plain", file=sys.stderr, flush=True)
+    lat_p, service, _ = serve_plain(gases, arrivals)
+    t_end = max(ta + l for (ta, _), l in zip(arrivals, lat_p))
+    out["plain"] = summarize(lat_p, 0.0, t_end)
+    out["plain_by_kind"] = {k: summarize(lat_p[kinds == k], 0.0, t_end)
+                            for k in ("wdot", "flamelet")}
+    out["plain_service_s_total"] = float(service.sum())
+    out["throughput_ratio"] = (out["batched"]["requests_per_s"]
+                               / out["plain"]["requests_per_s"])
+    out["p99_ratio_plain_over_batched"] = (out["plain"]["p99_s"]
+                                           / out["batched"]["p99_s"])
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -237,8 +287,11 @@ def main():
         print(json.dumps({"copy_probe_bytes": src.numel() * 8}), flush=True)
         return
     if args.no_wdot:
-        line = {"metric": METRIC, "n_gpus": 1, "data": "synthetic",
-                "flamelets": flamelet_leg(args, device)}
+        line = {"metric": METRIC, "n_gpus": 1, "data": "synthetic"}
+        if not args.no_flamelets:
+            line["flamelets"] = flamelet_leg(args, device)
+        if not args.no_serving:
+            line["serving"] = serving_leg(args, device)
         print(json.dumps(line), flush=True)
         return
     mech = bundled(args.mech)
@@ -315,6 +368,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_flamelets:
         line["flamelets"] = flamelet_leg(args, device)
+    if not args.no_serving:
+        line["serving"] = serving_leg(args, device)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(mech, args.cpu_seconds, args.seed)

@@ -345,156 +345,35 @@ class FlameletResult:
     finished_at: float = 0.0
 
 
-_NEWTON_TRY, _MARCH, _NEWTON, _DONE_OK, _DONE_FAIL, _EQUIL = range(6)
-
-
 def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
                 newton_maxiter=10, newton_tol=1e-8, bdf_newton_maxiter=20,
                 bdf_newton_tol=1e-9, bdf_time_step=1e-5, bdf_maxsteps=10,
                 try_newton=False, max_attempts=10, verbosity=False,
                 equilibrate_steps=0, equilibrate_dt0=1e-8,
                 equilibrate_growth=2.5, newton_norm="reference",
-                batch: Optional[FlameletBatch] = None) -> List[FlameletResult]:
+                kernel_events: Optional[dict] = None,
+                stats: Optional[dict] = None) -> List[FlameletResult]:
     """Solve every problem with the control flow of ``FlameletSolver.solve``
     applied to each flamelet independently (preceded, when
     ``equilibrate_steps`` > 0, by ``FlameletSolver.equilibrate``); one launch
-    sequence per iteration advances every unfinished flamelet.
-    ``newton_norm`` is the update norm of ``update_norm``."""
-    if newton_norm not in NEWTON_NORMS:
-        raise ValueError(f"unknown Newton norm {newton_norm!r}")
-    t_start = time.time()
-    B = len(problems)
-    b = batch or FlameletBatch(pyro_gas, [p.x for p in problems],
-                               np.stack([p.y_ox for p in problems]),
-                               np.stack([p.y_fu for p in problems]))
-    dev = b.dev
-    cat = lambda xs: torch.as_tensor(np.concatenate(xs, axis=-1),
-                                     dtype=torch.float64, device=dev)
-    phi = cat([np.asarray(p.phi) for p in problems]).contiguous()
-    phi_guess = phi.clone()
-    chi = cat([p.chi for p in problems])
-    vd = cat([p.vd if p.vd is not None else np.zeros(len(p.x))
-              for p in problems])
-    tg = cat([p.t_guess for p in problems])
-    pres = b.per_point([p.pressure for p in problems])
-    h_ox = np.array([p.h_ox for p in problems], dtype=float)
-    h_fu = np.array([p.h_fu for p in problems], dtype=float)
-    b.set_boundary_enthalpy(h_ox, h_fu)
-    h_scale = enthalpy_scale(h_ox, h_fu)
-    phi_prev = torch.empty_like(phi)
-    r_prev = torch.empty_like(phi)
-    r_tmp = torch.empty_like(phi)
-    phi_save = torch.empty_like(phi)
-    sys = b.sys.long()
-
-    first_phase = _NEWTON_TRY if try_newton else _MARCH
-    phase = np.full(B, _EQUIL if equilibrate_steps > 0 else first_phase)
-    chi0 = chi.clone()
-    interior = torch.ones(b.P, dtype=torch.bool, device=dev)
-    interior[b.off[:-1]] = False
-    interior[b.off[1:] - 1] = False
-    chi0[interior] = 0.0
-    eq_step = np.zeros(B, dtype=int)
-    it = np.zeros(B, dtype=int)
-    err_prev = np.full(B, np.inf)
-    step = np.zeros(B, dtype=int)
-    inner = np.zeros(B, dtype=int)
-    attempt = np.zeros(B, dtype=int)
-    n_newton = np.zeros(B, dtype=int)
-    n_steps = np.zeros(B, dtype=int)
-    finished = np.zeros(B)
-    b.dt.fill_(float(bdf_time_step))
-
-    def pts(mask_sys):
-        m = torch.as_tensor(mask_sys, dtype=torch.bool, device=dev)
-        return m[sys]
-
-    while True:
-        live = (phase != _DONE_OK) & (phase != _DONE_FAIL)
-        if not live.any():
-            break
-        eq = phase == _EQUIL
-        chi_eff = torch.where(pts(eq), chi0, chi) if eq.any() else chi
-        dts = np.where(eq, equilibrate_dt0 * equilibrate_growth ** eq_step,
-                       bdf_time_step)
-        b.dt.copy_(torch.as_tensor(dts, dtype=torch.float64))
-        cn = (phase == _MARCH) | eq
-        # flamelets starting a Crank-Nicolson step: phi_prev, R(phi_prev)
-        starting = cn & (inner == 0)
-        if starting.any():
-            b.residual(phi, chi_eff, vd, tg, pres, r_tmp)
-            pm = pts(starting)
-            phi_prev[:, pm] = phi[:, pm]
-            r_prev[:, pm] = r_tmp[:, pm]
-        modes = torch.as_tensor(cn.astype(np.int32), device=dev)
-        active = torch.as_tensor(live.astype(np.int32), device=dev)
-        norm2, _ = b.newton_update(phi, chi_eff, vd, tg, pres, modes, phi_prev,
-                                   r_prev, apply=True, active=active)
-        # refresh the temperature guess of live flamelets only, so a finished
-        # flamelet's final temperature does not depend on its neighbours
-        tg.copy_(torch.where(pts(live), b.T, tg))
-        delta = update_norm(norm2.cpu().numpy(), newton_norm, h_scale)
-        now = time.time() - t_start
-        newton = live & ((phase == _NEWTON_TRY) | (phase == _NEWTON))
-        n_newton += newton
-        ok = newton & (delta < newton_tol)
-        bad = newton & ~ok & ((err_prev < delta) | (it + 1 >= newton_maxiter))
-        cont = newton & ~ok & ~bad
-        phase[ok] = _DONE_OK
-        finished[ok] = now
-        err_prev[cont] = delta[cont]
-        it[cont] += 1
-        # failed first Newton: restart from the guess with a time march
-        f0 = bad & (phase == _NEWTON_TRY)
-        if f0.any():
-            pm = pts(f0)
-            phi[:, pm] = phi_guess[:, pm]
-            phase[f0], step[f0], inner[f0] = _MARCH, 0, 0
-        # failed Newton after a march: back to the marched state
-        f1 = bad & (phase == _NEWTON)
-        if f1.any():
-            pm = pts(f1)
-            phi[:, pm] = phi_save[:, pm]
-            attempt[f1] += 1
-            again = f1 & (attempt < max_attempts)
-            phase[again], step[again], inner[again] = _MARCH, 0, 0
-            give_up = f1 & ~again
-            phase[give_up] = _DONE_FAIL
-            finished[give_up] = now
-        # Crank-Nicolson inner iterations (time march and equilibration)
-        march = live & cn
-        inner[march] += 1
-        step_done = march & ((delta <= bdf_newton_tol)
-                             | (inner >= bdf_newton_maxiter))
-        inner[step_done] = 0
-        eq_done = step_done & eq
-        eq_step[eq_done] += 1
-        eq_finished = eq_done & (eq_step >= equilibrate_steps)
-        if eq_finished.any():
-            pm = pts(eq_finished)
-            phi_guess[:, pm] = phi[:, pm]
-            phase[eq_finished] = first_phase
-            it[eq_finished], err_prev[eq_finished] = 0, np.inf
-        step_done = step_done & ~eq
-        step[step_done] += 1
-        n_steps += step_done
-        to_newton = step_done & (step >= bdf_maxsteps)
-        if to_newton.any():
-            pm = pts(to_newton)
-            phi_save[:, pm] = phi[:, pm]
-            phase[to_newton] = _NEWTON
-            it[to_newton], err_prev[to_newton] = 0, np.inf
-        if verbosity:
-            print(f"batch iteration: live {int(live.sum())}, "
-                  f"max |v| {np.max(delta[live]):.3e}")
-
-    # final temperature of each converged state
-    b.source(phi, vd, tg, pres, jac=False)
-    out = []
-    for j, (a, e) in enumerate(zip(b.offsets_np[:-1], b.offsets_np[1:])):
-        st = _array_to_state(phi[:, a:e].clone())
-        out.append(FlameletResult(st, b.T[a:e].clone(),
-                                  bool(phase[j] == _DONE_OK), int(n_newton[j]),
-                                  int(n_steps[j]), int(attempt[j]),
-                                  float(finished[j])))
-    return out
+    sequence per iteration advances every unfinished flamelet
+    (``FlameletStepper``).  ``newton_norm`` is the update norm of
+    ``update_norm``; ``kernel_events``, if a dict, collects HIP events around
+    every kernel (``FlameletBatch.kernel_ms``); ``stats``, if a dict,
+    receives the iteration count and the sum of batch points over them."""
+    from .stepper import FlameletStepper, SolveSettings
+    settings = SolveSettings(
+        newton_maxiter=newton_maxiter, newton_tol=newton_tol,
+        bdf_newton_maxiter=bdf_newton_maxiter, bdf_newton_tol=bdf_newton_tol,
+        bdf_time_step=bdf_time_step, bdf_maxsteps=bdf_maxsteps,
+        try_newton=try_newton, max_attempts=max_attempts,
+        equilibrate_steps=equilibrate_steps, equilibrate_dt0=equilibrate_dt0,
+        equilibrate_growth=equilibrate_growth, newton_norm=newton_norm)
+    st = FlameletStepper(pyro_gas, settings, verbosity=verbosity,
+                         kernel_events=kernel_events)
+    handles = st.add(problems)
+    results = st.run()
+    if stats is not None:
+        stats.update(iterations=st.iterations,
+                     point_iterations=st.point_iterations)
+    return [results[h] for h in handles]

@@ -1,0 +1,300 @@
+"""Iteration-level scheduler for batches of flamelets whose membership can
+change between iterations (continuous batching).
+
+Every flamelet runs the control flow of the reference's hybrid solve
+(``flamelets/solver.py:476-600``: optional Newton try, Crank-Nicolson time
+march of ``bdf_maxsteps`` steps as globalisation, Newton, up to
+``max_attempts`` restarts), optionally preceded by an equilibration phase.
+``step()`` advances every live flamelet by one Newton update with one launch
+sequence (``FlameletBatch.newton_update``), whichever phase each one is in,
+then retires the flamelets that finished.  ``add()`` admits new flamelets;
+they join at the next step.  When membership changes the live flamelets are
+re-packed into a new contiguous batch; a flamelet's arithmetic never depends
+on its neighbours (per-point kernels, per-flamelet norms and control flow),
+so its result is the same bit for bit alone, in any batch, and whenever it
+joined.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from .engine import FlameletBatch
+
+NEWTON_TRY, MARCH, NEWTON, DONE_OK, DONE_FAIL, EQUIL = range(6)
+
+
+@dataclass
+class SolveSettings:
+    """The reference's ``solve`` arguments (``solver.py:476-492``) plus the
+    equilibration phase and the update norm."""
+    newton_maxiter: int = 10
+    newton_tol: float = 1e-8
+    bdf_newton_maxiter: int = 20
+    bdf_newton_tol: float = 1e-9
+    bdf_time_step: float = 1e-5
+    bdf_maxsteps: int = 10
+    try_newton: bool = False
+    max_attempts: int = 10
+    equilibrate_steps: int = 0
+    equilibrate_dt0: float = 1e-8
+    equilibrate_growth: float = 2.5
+    newton_norm: str = "reference"
+
+
+# per-flamelet integer/float control state, in this order
+_CTRL = ("phase", "it", "err_prev", "step", "inner", "attempt", "n_newton",
+         "n_steps", "eq_step")
+# per-flamelet device arrays: (nv, N) state-shaped and (N,) point-shaped
+_STATE = ("phi", "phi_guess", "phi_prev", "r_prev", "phi_save")
+_POINT = ("tg", "chi", "chi0", "vd", "pres")
+
+
+@dataclass
+class _Slot:
+    handle: int
+    problem: object
+    ctrl: Dict[str, float]
+    arrays: Dict[str, torch.Tensor]
+    t_added: float
+    finished_at: float = 0.0
+
+
+class FlameletStepper:
+    def __init__(self, gas, settings: Optional[SolveSettings] = None,
+                 verbosity=False, device=None,
+                 kernel_events: Optional[dict] = None):
+        from .solver import NEWTON_NORMS
+        self.gas = gas
+        self.s = settings or SolveSettings()
+        if self.s.newton_norm not in NEWTON_NORMS:
+            raise ValueError(f"unknown Newton norm {self.s.newton_norm!r}")
+        self.dev = torch.device(device) if device is not None else gas.device
+        self.nv = gas.num_species + 1
+        self.verbosity = verbosity
+        self._slots: List[_Slot] = []    # live flamelets, batch order
+        self._pending: List[_Slot] = []  # admitted, not yet packed
+        self._next = 0
+        self.batch: Optional[FlameletBatch] = None
+        self.iterations = 0
+        self.point_iterations = 0  # sum over iterations of the batch points
+        self.t0 = time.time()
+        # when a dict is given, every batch records HIP events into it
+        self.kernel_events = kernel_events
+
+    # -- membership -------------------------------------------------------
+    def add(self, problems: Sequence) -> List[int]:
+        """Admit flamelets (``FlameletProblem``); returns their handles."""
+        s, dev, dd = self.s, self.dev, dict(dtype=torch.float64,
+                                            device=self.dev)
+        first = NEWTON_TRY if s.try_newton else MARCH
+        out = []
+        for p in problems:
+            n = len(p.x)
+            phi = torch.as_tensor(np.asarray(p.phi), **dd).contiguous()
+            chi = torch.as_tensor(np.asarray(p.chi), **dd)
+            chi0 = chi.clone()
+            chi0[1:-1] = 0.0
+            arrays = {
+                "phi": phi, "phi_guess": phi.clone(),
+                "phi_prev": torch.empty_like(phi),
+                "r_prev": torch.empty_like(phi),
+                "phi_save": torch.empty_like(phi),
+                "tg": torch.as_tensor(np.asarray(p.t_guess), **dd).clone(),
+                "chi": chi, "chi0": chi0,
+                "vd": (torch.zeros(n, **dd) if p.vd is None
+                       else torch.as_tensor(np.asarray(p.vd), **dd)),
+                "pres": torch.full((n,), float(p.pressure), **dd),
+            }
+            ctrl = dict(phase=EQUIL if s.equilibrate_steps > 0 else first,
+                        it=0, err_prev=np.inf, step=0, inner=0, attempt=0,
+                        n_newton=0, n_steps=0, eq_step=0)
+            slot = _Slot(self._next, p, ctrl, arrays, time.time())
+            self._next += 1
+            self._pending.append(slot)
+            out.append(slot.handle)
+        return out
+
+    @property
+    def n_live(self) -> int:
+        return len(self._slots) + len(self._pending)
+
+    @property
+    def points(self) -> int:
+        return sum(len(sl.problem.x) for sl in self._slots + self._pending)
+
+    @staticmethod
+    def bytes_per_point(nv: int) -> int:
+        """Device bytes a flamelet point holds while it is live."""
+        return 8 * (nv * nv + nv * (nv + 1) + (len(_STATE) + 4) * nv
+                    + len(_POINT) + 10)
+
+    def _unpack(self):
+        if self.batch is None:
+            return
+        for j, sl in enumerate(self._slots):
+            a, e = int(self.off[j]), int(self.off[j + 1])
+            for k in _STATE:
+                sl.arrays[k] = self.A[k][:, a:e].clone()
+            for k in _POINT:
+                sl.arrays[k] = self.A[k][a:e].clone()
+            for k in _CTRL:
+                sl.ctrl[k] = self.C[k][j]
+        self.batch = None
+
+    def _pack(self):
+        self._unpack()
+        self._slots = [sl for sl in self._slots
+                       if sl.ctrl["phase"] not in (DONE_OK, DONE_FAIL)]
+        self._slots += self._pending
+        self._pending = []
+        if not self._slots:
+            return
+        probs = [sl.problem for sl in self._slots]
+        b = FlameletBatch(self.gas, [p.x for p in probs],
+                          np.stack([p.y_ox for p in probs]),
+                          np.stack([p.y_fu for p in probs]), device=self.dev)
+        h_ox = np.array([p.h_ox for p in probs], dtype=float)
+        h_fu = np.array([p.h_fu for p in probs], dtype=float)
+        b.set_boundary_enthalpy(h_ox, h_fu)
+        b.dt.fill_(float(self.s.bdf_time_step))
+        if self.kernel_events is not None:
+            b.events = self.kernel_events
+        from .solver import enthalpy_scale
+        self.h_scale = enthalpy_scale(h_ox, h_fu)
+        self.batch = b
+        self.off = b.offsets_np
+        self.A = {k: torch.cat([sl.arrays[k] for sl in self._slots], dim=1)
+                  .contiguous() for k in _STATE}
+        self.A.update({k: torch.cat([sl.arrays[k] for sl in self._slots])
+                       for k in _POINT})
+        self.C = {k: np.array([sl.ctrl[k] for sl in self._slots],
+                              dtype=float if k == "err_prev" else np.int64)
+                  for k in _CTRL}
+        self.r_tmp = torch.empty_like(self.A["phi"])
+        self.sys = b.sys.long()
+        for sl in self._slots:  # the packed arrays now own the state
+            sl.arrays = {}
+
+    # -- one iteration ----------------------------------------------------
+    def _pts(self, mask_sys):
+        m = torch.as_tensor(mask_sys, dtype=torch.bool, device=self.dev)
+        return m[self.sys]
+
+    def step(self):
+        """Advance every live flamelet by one update; returns the list of
+        ``(handle, FlameletResult)`` of the flamelets that finished."""
+        from .solver import FlameletResult, update_norm
+        from .state import _array_to_state
+        if self._pending or self.batch is None:
+            self._pack()
+        if self.batch is None:
+            return []
+        s, b, A, C = self.s, self.batch, self.A, self.C
+        phase = C["phase"]
+        phi, tg, chi, chi0, vd, pres = (A["phi"], A["tg"], A["chi"],
+                                        A["chi0"], A["vd"], A["pres"])
+        live = (phase != DONE_OK) & (phase != DONE_FAIL)
+        eq = phase == EQUIL
+        chi_eff = torch.where(self._pts(eq), chi0, chi) if eq.any() else chi
+        dts = np.where(eq, s.equilibrate_dt0 * s.equilibrate_growth
+                       ** C["eq_step"], s.bdf_time_step)
+        b.dt.copy_(torch.as_tensor(dts, dtype=torch.float64))
+        cn = (phase == MARCH) | eq
+        starting = cn & (C["inner"] == 0)
+        if starting.any():
+            b.residual(phi, chi_eff, vd, tg, pres, self.r_tmp)
+            pm = self._pts(starting)
+            A["phi_prev"][:, pm] = phi[:, pm]
+            A["r_prev"][:, pm] = self.r_tmp[:, pm]
+        modes = torch.as_tensor(cn.astype(np.int32), device=self.dev)
+        active = torch.as_tensor(live.astype(np.int32), device=self.dev)
+        norm2, _ = b.newton_update(phi, chi_eff, vd, tg, pres, modes,
+                                   A["phi_prev"], A["r_prev"], apply=True,
+                                   active=active)
+        # refresh the temperature guess of live flamelets only
+        tg.copy_(torch.where(self._pts(live), b.T, tg))
+        delta = update_norm(norm2.cpu().numpy(), s.newton_norm, self.h_scale)
+        self.iterations += 1
+        self.point_iterations += b.P
+        now = time.time()
+        it, err_prev = C["it"], C["err_prev"]
+        newton = live & ((phase == NEWTON_TRY) | (phase == NEWTON))
+        C["n_newton"] += newton
+        ok = newton & (delta < s.newton_tol)
+        bad = newton & ~ok & ((err_prev < delta) | (it + 1 >= s.newton_maxiter))
+        cont = newton & ~ok & ~bad
+        phase[ok] = DONE_OK
+        err_prev[cont] = delta[cont]
+        it[cont] += 1
+        f0 = bad & (phase == NEWTON_TRY)
+        if f0.any():
+            pm = self._pts(f0)
+            phi[:, pm] = A["phi_guess"][:, pm]
+            phase[f0], C["step"][f0], C["inner"][f0] = MARCH, 0, 0
+        f1 = bad & (phase == NEWTON)
+        if f1.any():
+            pm = self._pts(f1)
+            phi[:, pm] = A["phi_save"][:, pm]
+            C["attempt"][f1] += 1
+            again = f1 & (C["attempt"] < s.max_attempts)
+            phase[again], C["step"][again], C["inner"][again] = MARCH, 0, 0
+            phase[f1 & ~again] = DONE_FAIL
+        march = live & cn
+        C["inner"][march] += 1
+        step_done = march & ((delta <= s.bdf_newton_tol)
+                             | (C["inner"] >= s.bdf_newton_maxiter))
+        C["inner"][step_done] = 0
+        eq_done = step_done & eq
+        C["eq_step"][eq_done] += 1
+        eq_finished = eq_done & (C["eq_step"] >= s.equilibrate_steps)
+        if eq_finished.any():
+            pm = self._pts(eq_finished)
+            A["phi_guess"][:, pm] = phi[:, pm]
+            phase[eq_finished] = NEWTON_TRY if s.try_newton else MARCH
+            it[eq_finished], err_prev[eq_finished] = 0, np.inf
+        step_done = step_done & ~eq
+        C["step"][step_done] += 1
+        C["n_steps"] += step_done
+        to_newton = step_done & (C["step"] >= s.bdf_maxsteps)
+        if to_newton.any():
+            pm = self._pts(to_newton)
+            A["phi_save"][:, pm] = phi[:, pm]
+            phase[to_newton] = NEWTON
+            it[to_newton], err_prev[to_newton] = 0, np.inf
+        if self.verbosity:
+            print(f"batch iteration: live {int(live.sum())}, "
+                  f"max |v| {np.max(delta[live]):.3e}")
+        done = live & ((phase == DONE_OK) | (phase == DONE_FAIL))
+        if not done.any():
+            return []
+        # final temperature of the finished flamelets from their last guess
+        b.source(phi, vd, tg, pres, jac=False)
+        out = []
+        for j in np.nonzero(done)[0]:
+            sl = self._slots[j]
+            a, e = int(self.off[j]), int(self.off[j + 1])
+            sl.finished_at = now
+            st = _array_to_state(phi[:, a:e].clone())
+            out.append((sl.handle, FlameletResult(
+                st, b.T[a:e].clone(), bool(phase[j] == DONE_OK),
+                int(C["n_newton"][j]), int(C["n_steps"][j]),
+                int(C["attempt"][j]), now - sl.t_added)))
+        if done.sum() < live.sum():
+            self._pack()  # drop the finished flamelets
+        else:
+            self._slots, self.batch = [], None
+        return out
+
+    def run(self) -> Dict[int, object]:
+        """Step until every admitted flamelet has finished."""
+        results = {}
+        while self.n_live:
+            for h, r in self.step():
+                results[h] = r
+        return results
