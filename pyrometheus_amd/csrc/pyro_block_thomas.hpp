@@ -79,6 +79,80 @@ __device__ __forceinline__ void gauss_jordan(double (&a)[W], int lane,
   }
 }
 
+// Max over lanes 0..15 of a double, by DPP rotations within the first
+// 16-lane row (lanes 16..63 rotate among themselves and are ignored).
+__device__ __forceinline__ double row16_max(double v) {
+#define PYRO_ROR_MAX(N)                                                      \
+  {                                                                          \
+    const long long b = __double_as_longlong(v);                             \
+    const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b), 0x120 + N,  \
+                                            0xF, 0xF, false);                \
+    const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32),       \
+                                            0x120 + N, 0xF, 0xF, false);     \
+    v = fmax(v, __longlong_as_double((static_cast<long long>(hi) << 32) |    \
+                                     static_cast<unsigned int>(lo)));        \
+  }
+  PYRO_ROR_MAX(8)
+  PYRO_ROR_MAX(4)
+  PYRO_ROR_MAX(2)
+  PYRO_ROR_MAX(1)
+#undef PYRO_ROR_MAX
+  return v;
+}
+
+// Gauss-Jordan for NV <= 16 with the same result layout as gauss_jordan.
+// Each lane first scales its row so that its largest |C'| entry is 1 (the
+// solution does not change; pivoting becomes scaled partial pivoting).  The
+// pivot is found by one DPP max-reduction of |a[k]| with the lane index
+// packed into the low mantissa bits, the reciprocal of the pivot by
+// v_rcp_f64 and two Newton steps, and every lane -- the pivot lane included
+// -- updates its row with one FMA per column: the pivot lane uses the
+// multiplier 1 - 1/piv, which turns a - (1 - 1/piv) a into a/piv with a
+// relative error of |piv - 1| eps, small because the rows are scaled.
+template <int NV, int W>
+__device__ __forceinline__ void gauss_jordan16(double (&a)[W], int lane,
+                                               int& my_k, int (&perm)[NV],
+                                               bool& singular) {
+  static_assert(NV <= 16, "one DPP row holds the pivot candidates");
+  const bool active = lane < NV;
+  {
+    double mx = 0.0;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) mx = fmax(mx, fabs(a[c]));
+    const double sc = mx > 0.0 ? pyro::recip(mx) : 1.0;
+#pragma unroll
+    for (int c = 0; c < W; ++c) a[c] *= sc;
+  }
+  unsigned used = 0u;
+  my_k = -1;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double key = -1.0;
+    if (active && !((used >> lane) & 1u)) {
+      const double v = fabs(a[k]);
+      key = (v == v) ? v : -0.5;  // a NaN row is taken only as a last resort
+    }
+    long long kb = __double_as_longlong(key);
+    if (key >= 0.0) kb = (kb & ~0xFll) | (15 - lane);
+    const double best = bcast(row16_max(__longlong_as_double(kb)), 0);
+    int p;
+    if (best >= 0.0)
+      p = 15 - static_cast<int>(__double_as_longlong(best) & 0xF);
+    else
+      p = __builtin_ctz(~used & ((1u << NV) - 1u));
+    p = __builtin_amdgcn_readfirstlane(p);
+    used |= 1u << p;
+    perm[k] = p;
+    const double piv = bcast(a[k], p);
+    singular = singular || !(fabs(piv) > 0.0);
+    const double inv = pyro::recip(piv);
+    const double g = (lane == p) ? 1.0 - inv : a[k] * inv;
+#pragma unroll
+    for (int j = k + 1; j < W; ++j) a[j] = fma(-g, bcast(a[j], p), a[j]);
+    if (lane == p) my_k = k;
+  }
+}
+
 template <int NV, bool DIAG>
 __global__ void __launch_bounds__(64)
 k_block_thomas(const double* __restrict__ L, const double* __restrict__ C,
@@ -115,7 +189,23 @@ k_block_thomas(const double* __restrict__ L, const double* __restrict__ C,
   }
   for (int64_t i = 0; i < n; ++i) {
     const int64_t g = off + i;
-    gauss_jordan<NV, W>(a, lane, my_k, perm, singular);
+    // block row i+1 of C and r, row = lane, loaded before the elimination
+    // so that its latency overlaps it (the flamelet form)
+    double cpre[NV], rpre = 0.0;
+    if constexpr (DIAG) {
+      if (i + 1 < n) {
+        const int64_t g1 = g + 1;
+        const int r = active ? lane : 0;
+#pragma unroll
+        for (int c = 0; c < NV; ++c) cpre[c] = C[(g1 * NV + r) * NV + c];
+        rpre = R[g1 * NV + r];
+      }
+    }
+    if constexpr (NV <= 16) {
+      gauss_jordan16<NV, W>(a, lane, my_k, perm, singular);
+    } else {
+      gauss_jordan<NV, W>(a, lane, my_k, perm, singular);
+    }
     // store logical row my_k of [C'^-1 U | C'^-1 r']
     if (active) {
       double* wrow = work + (g * NV + my_k) * (NV + 1);
@@ -130,11 +220,13 @@ k_block_thomas(const double* __restrict__ L, const double* __restrict__ C,
     const int k = active ? my_k : 0;
     double nxt[W];
     if constexpr (DIAG) {
+      // this lane continues with logical row k: take row k of the
+      // prefetched block from the lane that loaded it
       const double l = L[g];
 #pragma unroll
       for (int c = 0; c < NV; ++c)
-        nxt[c] = fma(-l, a[NV + c], C[(g1 * NV + k) * NV + c]);
-      nxt[2 * NV] = fma(-l, a[2 * NV], R[g1 * NV + k]);
+        nxt[c] = fma(-l, a[NV + c], __shfl(cpre[c], k));
+      nxt[2 * NV] = fma(-l, a[2 * NV], __shfl(rpre, k));
     } else {
 #pragma unroll
       for (int c = 0; c < NV; ++c) nxt[c] = C[(g1 * NV + k) * NV + c];
@@ -173,15 +265,31 @@ k_block_thomas(const double* __restrict__ L, const double* __restrict__ C,
 #pragma unroll
     for (int m = 0; m < NV; ++m) x_next[m] = bcast(xr, m);
   }
+  double wcur[NV + 1];
+  if (n >= 2) {
+    const double* wrow = work + ((off + n - 2) * NV + r) * (NV + 1);
+#pragma unroll
+    for (int m = 0; m <= NV; ++m) wcur[m] = wrow[m];
+  }
   for (int64_t i = n - 2; i >= 0; --i) {
     const int64_t g = off + i;
-    const double* wrow = work + (g * NV + r) * (NV + 1);
-    double xr = wrow[NV];
+    // next row's workspace entries, loaded while this row is reduced
+    double wnext[NV + 1];
+    if (i > 0) {
+      const double* wrow = work + ((g - 1) * NV + r) * (NV + 1);
 #pragma unroll
-    for (int m = 0; m < NV; ++m) xr = fma(-wrow[m], x_next[m], xr);
+      for (int m = 0; m <= NV; ++m) wnext[m] = wrow[m];
+    }
+    double xr = wcur[NV];
+#pragma unroll
+    for (int m = 0; m < NV; ++m) xr = fma(-wcur[m], x_next[m], xr);
     if (active) X[g * NV + r] = xr;
 #pragma unroll
     for (int m = 0; m < NV; ++m) x_next[m] = bcast(xr, m);
+    if (i > 0) {
+#pragma unroll
+      for (int m = 0; m <= NV; ++m) wcur[m] = wnext[m];
+    }
   }
   if (singular && lane == 0 && n_singular) atomicAdd(n_singular, 1);
 }

@@ -120,13 +120,21 @@ class FlameletStepper:
             out.append(slot.handle)
         return out
 
+    def _live_slots(self):
+        if self.batch is None:
+            return list(self._slots)
+        ph = self.C["phase"]
+        return [sl for j, sl in enumerate(self._slots)
+                if ph[j] not in (DONE_OK, DONE_FAIL)]
+
     @property
     def n_live(self) -> int:
-        return len(self._slots) + len(self._pending)
+        return len(self._live_slots()) + len(self._pending)
 
     @property
     def points(self) -> int:
-        return sum(len(sl.problem.x) for sl in self._slots + self._pending)
+        return sum(len(sl.problem.x)
+                   for sl in self._live_slots() + self._pending)
 
     @staticmethod
     def bytes_per_point(nv: int) -> int:
@@ -285,10 +293,15 @@ class FlameletStepper:
                 st, b.T[a:e].clone(), bool(phase[j] == DONE_OK),
                 int(C["n_newton"][j]), int(C["n_steps"][j]),
                 int(C["attempt"][j]), now - sl.t_added)))
-        if done.sum() < live.sum():
-            self._pack()  # drop the finished flamelets
-        else:
+        if done.sum() == live.sum():
             self._slots, self.batch = [], None
+        else:
+            # finished flamelets stay in the batch, masked out of every
+            # update, until they hold half its points (or flamelets join)
+            dead = ~live | done
+            sizes = np.diff(self.off)
+            if sizes[dead].sum() * 2 > sizes.sum():
+                self._pack()
         return out
 
     def run(self) -> Dict[int, object]:

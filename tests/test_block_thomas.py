@@ -104,3 +104,41 @@ def test_gpu_drop_in_signature(device):
     t = lambda a: torch.tensor(a, dtype=torch.float64, device=device)
     x = bt(t(L), t(C), t(U), t(r)).cpu().numpy()
     assert np.max(np.abs(x - block_thomas(L, C, U, r))) < 1e-12 * np.max(np.abs(x))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("diag", [False, True])
+def test_gpu_badly_scaled_rows(diag, device):
+    """Rows scaled by factors from 1e-8 to 1e8, as in the flamelet Jacobian
+    whose enthalpy row is in J/kg.  Dense couplings: the row-scaled system
+    has the solution of the unscaled one, to 1e-12.  Scalar couplings (which
+    cannot carry a row scale): badly scaled central blocks, against the
+    oracle on the same system, to 1e-10."""
+    import torch
+    from pyrometheus_amd.flamelets.linear_solver import block_thomas_batched
+    rng = np.random.default_rng(77)
+    nv, n = 10, 40
+    L, C, U, r = _system(rng, n, nv, pivot_stress=True)
+    d = 10.0 ** rng.uniform(-8, 8, size=(n, nv))
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=device)
+    offs = torch.tensor([0, n], device=device)
+    if diag:
+        lv, uv = rng.normal(size=n - 1), rng.normal(size=n - 1)
+        L = lv[:, None, None] * np.eye(nv)
+        U = uv[:, None, None] * np.eye(nv)
+        Cs = C * d[:, :, None]
+        ref = block_thomas(L, Cs, U, r)
+        x = block_thomas_batched(t(np.append(lv, 0.0)), t(Cs),
+                                 t(np.append(uv, 0.0)), t(r), offs,
+                                 diag=True).cpu().numpy()
+        tol = 1e-10
+    else:
+        ref = block_thomas(L, C, U, r)
+        Lf, Uf = np.zeros((n, nv, nv)), np.zeros((n, nv, nv))
+        Lf[:n - 1] = L * d[1:, :, None]
+        Uf[:n - 1] = U * d[:-1, :, None]
+        x = block_thomas_batched(t(Lf), t(C * d[:, :, None]), t(Uf),
+                                 t(r * d), offs, diag=False).cpu().numpy()
+        tol = 1e-12
+    err = np.max(np.abs(x - ref)) / np.max(np.abs(ref))
+    assert err < tol, err
