@@ -228,9 +228,10 @@ def serving_leg(args, device):
     gases = {n: get_thermochem_class(bundled(n))(device=device)
              for n in ("sandiego", "gri30")}
     arrivals = poisson_workload(gases, args.serve_requests, args.serve_rate,
-                                seed=5)
+                                seed=5, flamelet_mechanisms=["sandiego"])
     warm = poisson_workload(gases, 4, 100.0, seed=6, wdot_log2_cells=(10, 12),
-                            flamelet_points=(64, 96))
+                            flamelet_points=(64, 96),
+                            flamelet_mechanisms=["sandiego"])
     ChemistryServer(gases).run(warm)
     print(f"[bench] serving: {len(arrivals)} requests at "
           f"{args.serve_rate}/s", file=sys.stderr, flush=True)

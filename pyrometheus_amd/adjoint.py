@@ -12,7 +12,9 @@ lane and reaction-major like the primal kernel:
 
 and ``pyro_source_jacobian`` gives the full (ns+1) x (ns+1) Jacobian of the
 flamelet source S = [vd/ρ; W_k ω̇_k/ρ] with respect to φ = [h; Y]
-(``equations.py:361-389``), one forward-mode column per pass.
+(``equations.py:361-389``) in two passes: one stores per reaction k_f, k_r,
+q, ∂q/∂T and ∂q/∂[M] to a workspace, the other assembles each row from the
+reactions of its species and the chain rule through T(h, Y) and ρ(T, Y).
 
 ``NetProductionRates`` wraps the pair in a ``torch.autograd.Function`` so
 ω̇ takes part in torch's reverse mode (``backward`` is the VJP kernel) and
@@ -80,7 +82,8 @@ def wdot_vjp(gas, rho, temperature, mass_fractions, wbar):
 
 def source_jacobian(gas, temperature, pressure, mass_fractions, vd=None,
                     layout: str = "cell", out: Optional[torch.Tensor] = None,
-                    source: Optional[torch.Tensor] = None):
+                    source: Optional[torch.Tensor] = None,
+                    work: Optional[torch.Tensor] = None):
     """∂S/∂φ per cell for S = [vd/ρ; W_k ω̇_k/ρ], φ = [h; Y], ρ = pW̄/(RT),
     at a temperature consistent with h.
 
@@ -88,7 +91,9 @@ def source_jacobian(gas, temperature, pressure, mass_fractions, vd=None,
     block layout (``vmap(jacfwd(...))``, ``equations.py:124-129``).
     ``layout="entry"``: an (nv, nv, N) tensor, entry-major, which the kernel
     writes with fully coalesced stores.  ``source``, if given, an (nv, N)
-    tensor, receives S."""
+    tensor, receives S.  Cells are processed in fixed chunks through a
+    workspace of ``pyro_source_jacobian_work_doubles()`` doubles (the
+    per-reaction cache of one chunk); pass ``work`` to reuse one."""
     y, cs = gas._cells(mass_fractions)
     n = y.shape[1]
     t = gas._temperature_field(temperature, n, cs)
@@ -108,12 +113,24 @@ def source_jacobian(gas, temperature, pressure, mass_fractions, vd=None,
     if source is not None and (tuple(source.shape) != (nv, n)
                                or source.stride(1) != 1):
         raise ValueError(f"source must be an ({nv}, {n}) tensor")
+    work = source_jacobian_workspace(gas, work)
     gas._lib.call("pyro_source_jacobian", t.data_ptr(), p.data_ptr(), ps,
                   y.data_ptr(), y.stride(0), _ptr(v), out.data_ptr(), s_rc,
                   s_i, _ptr(source),
-                  source.stride(0) if source is not None else n, n,
-                  _stream())
+                  source.stride(0) if source is not None else n,
+                  work.data_ptr(), n, _stream())
     return out
+
+
+def source_jacobian_workspace(gas, work: Optional[torch.Tensor] = None):
+    """The workspace ``source_jacobian`` needs (checked, or allocated)."""
+    need = int(gas._lib.call("pyro_source_jacobian_work_doubles"))
+    if work is None:
+        return torch.empty(need, dtype=torch.float64, device=gas.device)
+    if work.dtype != torch.float64 or work.numel() < need or \
+            not work.is_contiguous():
+        raise ValueError(f"workspace must hold {need} contiguous doubles")
+    return work
 
 
 class NetProductionRates(torch.autograd.Function):

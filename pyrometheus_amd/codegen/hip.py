@@ -869,6 +869,177 @@ def _emit_wdot_vjp(w: _Writer, L: _Lowering):
     w("")
 
 
+def cache_slots(L: _Lowering):
+    """Slot of each cached per-reaction quantity in the source-Jacobian
+    workspace: k_f, k_r = k_f/K_c (reversible), q, dq/dT, dq/d[M]
+    (three-body and falloff).  Returns ({(i, name): slot}, n_slots)."""
+    slots = {}
+    for i, r in enumerate(L.m.reactions):
+        names = ["kf"] + (["kr"] if r.reversible else []) + ["q", "dqT"]
+        if r.kind in ("three-body", "falloff"):
+            names.append("dqM")
+        for nm in names:
+            slots[(i, nm)] = len(slots)
+    return slots, len(slots)
+
+
+def _emit_reaction_cache(w: _Writer, L: _Lowering):
+    """First pass of the source Jacobian: per reaction, the quantities the
+    row pass needs, stored to the workspace column of this cell."""
+    m = L.m
+    slots, nslots = cache_slots(L)
+    w(f"constexpr int kCacheSlots = {nslots};")
+    w("// cells per workspace chunk: the leading dimension of every slot is a")
+    w("// compile-time constant, so slot addresses are a uniform base plus a")
+    w("// constant and one per-lane offset shared by all slots")
+    w("constexpr int kCacheLd = 65536;")
+    w("// per-reaction k_f, k_r, q, dq/dT, dq/d[M] to the workspace")
+    w("__device__ __forceinline__ void reaction_cache(const double T, "
+      "const double (&C)[NS], double* __restrict__ work, const int il) {")
+    w.ind += 1
+    _deriv_prologue(w, L)
+    for i, r in enumerate(m.reactions):
+        w(f"{{  // R{i}: {r.equation} [{r.reaction_type}]")
+        w.ind += 1
+        L.kf_block_deriv(w, r)
+        _reverse_parts(w, L, r)
+        def at(nm):
+            return f"work[{slots[(i, nm)]} * kCacheLd + il]"
+        w(f"{at('kf')} = kf;")
+        if r.reversible:
+            w(f"{at('kr')} = kf * kinv;")
+            w(f"{at('dqT')} = fma(dkfT, delta, -kf * dkinvT * pb);")
+        else:
+            w(f"{at('dqT')} = dkfT * delta;")
+        w(f"{at('q')} = kf * delta;")
+        if r.kind in ("three-body", "falloff"):
+            w(f"{at('dqM')} = dkfM * delta;")
+        if (i + 1) % L.tuning.get("pin_every", 1) == 0:
+            w('asm volatile("" ::: "memory");')
+        w.ind -= 1
+        w("}")
+    w.ind -= 1
+    w("}")
+    w("")
+
+
+def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
+    """Second pass: row k+1 of dS/d[h; Y] from the cached quantities of the
+    reactions that involve species k.  G_kj = dw_k/dC_j is accumulated from
+    k_f dpf/dC_j - k_r dpb/dC_j (+ the [M] term); then, with
+    A_k = sum_j G_kj C_j / rho and B_k = dw_k/dT,
+      J[1+k][c] = W_k/rho ((A_k - w_k/rho) drho_c + B_k dT_c)
+                  + [c = 1+m] (W_k/W_m) G_km.
+    dT_c comes from LDS (dTl[m*hs] = dT of column 1+m)."""
+    m = L.m
+    slots, _ = cache_slots(L)
+    ns = L.ns
+    w("// rows of the source Jacobian from the reaction cache")
+    w("__device__ __forceinline__ void source_jacobian_rows("
+      "double (&C)[NS], const double rho, const double T, "
+      "const double cp, const double mmw, const double vd, "
+      "const double* __restrict__ work, const int il, "
+      "const double* __restrict__ dTl, const int hs, "
+      "double* __restrict__ o, const int64_t s_rc, "
+      "double* __restrict__ S, const int64_t ldS) {")
+    w.ind += 1
+    w("double irho = 1.0 / rho;")
+    w("double rT = rho * (1.0 / T);")
+    w("double dT0 = 1.0 / cp;")
+    w("double dr0 = -rT * dT0;")
+    w("double rm = -rho * mmw;  // drho_m = -rT dT_m + rm / W_m")
+    w(f"constexpr int NV = {ns + 1};")
+    w("{  // row 0: d(vd/rho)")
+    w.ind += 1
+    w("const double v0 = -vd * irho * irho;")
+    w("o[0] = v0 * dr0;")
+    w("#pragma unroll")
+    w("for (int q = 0; q < NS; ++q) {")
+    w("  const double dT = dTl[q * hs];")
+    w("  o[(1 + q) * s_rc] = v0 * fma(-rT, dT, rm * kInvW[q]);")
+    w("}")
+    w("if (S) S[0] = vd * irho;")
+    w.ind -= 1
+    w("}")
+    # reactions involving each species, with the net coefficient
+    inv = {k: [] for k in range(ns)}
+    for i, r in enumerate(m.reactions):
+        for k, nu in L.species_updates(r, "q"):
+            inv[k].append((i, nu))
+    for k in range(ns):
+        w(f"{{  // row {k + 1}: {m.species[k].name}")
+        w.ind += 1
+        # fresh copies of C and of the per-cell scalars for this row:
+        # products of them are then not shared with (and kept live for)
+        # other rows
+        for a in range(0, ns, 24):
+            ops = ", ".join(f'"+v"(C[{j}])' for j in range(a, min(ns, a + 24)))
+            w(f'asm volatile("" : {ops});')
+        w('asm volatile("" : "+v"(rm), "+v"(rT), "+v"(dr0), "+v"(dT0), '
+          '"+v"(irho));')
+        w("double G[NS];")
+        w("#pragma unroll")
+        w("for (int j = 0; j < NS; ++j) G[j] = 0.0;")
+        w("double wk = 0.0, bk = 0.0, dm = 0.0;")
+        for i, nu in inv[k]:
+            r = m.reactions[i]
+            nl = lit(nu)
+            w(f"{{  // R{i}: {r.equation}")
+            w.ind += 1
+            def at(nm, i=i):
+                return f"work[{slots[(i, nm)]} * kCacheLd + il]"
+            w(f"const double kf = {at('kf')};")
+            w(f"wk = fma({nl}, {at('q')}, wk);")
+            w(f"bk = fma({nl}, {at('dqT')}, bk);")
+            rpairs = L.reactant_pairs(r)
+            dpf = dict(L.prod_partials(rpairs))
+            dpb = {}
+            if r.reversible:
+                w(f"const double kr = {at('kr')};")
+                ppairs = [(L.idx[sp], nn) for sp, nn in r.products]
+                dpb = dict(L.prod_partials(ppairs))
+            w(f"const double a = {nl} * kf;")
+            if dpb:
+                w(f"const double b = {lit(-nu)} * kr;")
+            for j in sorted(set(dpf) | set(dpb)):
+                if j in dpf:
+                    w(f"G[{j}] = fma(a, {dpf[j]}, G[{j}]);")
+                if j in dpb:
+                    w(f"G[{j}] = fma(b, {dpb[j]}, G[{j}]);")
+            if r.kind in ("three-body", "falloff"):
+                w(f"const double mq = {nl} * {at('dqM')};")
+                for j, e in L.efficiency_terms(r):
+                    w(f"G[{j}] = fma({lit(e)}, mq, G[{j}]);")
+                if r.default_efficiency != 0.0:
+                    w(f"dm = fma({lit(r.default_efficiency)}, mq, dm);")
+            w.ind -= 1
+            w("}")
+        w("double A = 0.0;")
+        w("#pragma unroll")
+        w("for (int j = 0; j < NS; ++j) { G[j] += dm; A = fma(G[j], C[j], A); }")
+        w(f"const double f = kW[{k}] * irho;")
+        w("const double al = f * fma(-wk, irho, A * irho);")
+        w("const double be = f * bk;")
+        w(f"double* row = o + {(k + 1) * (ns + 1)} * s_rc;")
+        w("row[0] = fma(al, dr0, be * dT0);")
+        w("#pragma unroll")
+        w("for (int q = 0; q < NS; ++q) {")
+        w("  const double dT = dTl[q * hs];")
+        w("  const double dr = fma(-rT, dT, rm * kInvW[q]);")
+        w(f"  row[(1 + q) * s_rc] = fma(al, dr, fma(be, dT, "
+          f"kW[{k}] * kInvW[q] * G[q]));")
+        w("}")
+        w(f"if (S) S[{k + 1} * ldS] = f * wk;")
+        # keep the next row's loads below this row's stores: one row's
+        # accumulators live at a time
+        w('asm volatile("" ::: "memory");')
+        w.ind -= 1
+        w("}")
+    w.ind -= 1
+    w("}")
+    w("")
+
+
 def _emit_reaction_quantities(w: _Writer, L: _Lowering):
     m = L.m
     w("// per-reaction k_f, -ln K_c, r_f, r_r, r_net to memory "
@@ -979,14 +1150,17 @@ class HipCodeGenerator(CodeGenerator):
         _emit_creation_destruction(w, L)
         _emit_wdot_jvp(w, L)
         _emit_wdot_vjp(w, L)
+        _emit_reaction_cache(w, L)
+        _emit_source_jacobian_rows(w, L)
         dense = mech.n_species <= DENSE_JAC_MAX_SPECIES
         if dense:
             _emit_wdot_jac(w, L)
         w("}  // namespace mech")
         w("")
         w('#include "pyro_thermo_kernels.hpp"')
+        w('#include "pyro_flamelet_kernels.hpp"')
         if dense:
-            w('#include "pyro_flamelet_kernels.hpp"')
+            w('#include "pyro_flamelet_dense.hpp"')
         return w.text()
 
     @staticmethod

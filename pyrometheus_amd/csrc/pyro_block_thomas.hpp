@@ -10,10 +10,11 @@
 //
 // Mapping: one wave64 per system.  Lane l (< NV) owns one logical row of the
 // augmented block [C' | U | r'] (2 NV + 1 doubles in registers).  Each block
-// row is reduced by Gauss-Jordan elimination with partial pivoting in which
-// the pivot row is never moved: the lane chosen as pivot for column k simply
-// takes logical row k ("virtual pivoting"), the pivot row is broadcast with
-// v_readlane, and every other lane eliminates with its own column-k entry.
+// row is reduced by Gauss-Jordan elimination with scaled partial pivoting in
+// which the pivot row is never moved: the lane chosen as pivot for column k
+// simply takes logical row k ("virtual pivoting"), the pivot row is
+// broadcast with v_readlane, and every lane eliminates with its own
+// column-k entry.
 // The result [I | C'^-1 U | C'^-1 r'] is stored to a workspace for the back
 // substitution, and each lane builds its logical row of the next block from
 // the row it already holds, so no row data ever crosses lanes.  With DIAG the
@@ -38,47 +39,6 @@ __device__ __forceinline__ double bcast(double v, int lane) {
                               static_cast<unsigned int>(lo));
 }
 
-// Gauss-Jordan on the augmented rows a[0..W) held one per lane (lanes < NV).
-// On return the lane's row is logical row `my_k` of [I | X].  `perm[k]` is
-// the lane holding logical row k (wave-uniform).
-template <int NV, int W>
-__device__ __forceinline__ void gauss_jordan(double (&a)[W], int lane,
-                                             int& my_k, int (&perm)[NV],
-                                             bool& singular) {
-  unsigned long long used = 0ull;
-  my_k = -1;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    // partial pivoting: largest |a[k]| among lanes not yet used
-    double best = -1.0;
-    int p = -1;
-#pragma unroll
-    for (int r = 0; r < NV; ++r) {
-      const double v = bcast(fabs(a[k]), r);
-      const bool free_r = ((used >> r) & 1ull) == 0ull;
-      if (free_r && (p < 0 || v > best)) {  // NaN rows still get a pivot slot
-        best = v;
-        p = r;
-      }
-    }
-    p = __builtin_amdgcn_readfirstlane(p);
-    used |= 1ull << p;
-    perm[k] = p;
-    const double piv = bcast(a[k], p);
-    singular = singular || !(fabs(piv) > 0.0);
-    const double inv = 1.0 / piv;
-    const bool is_p = (lane == p);
-    const double f = a[k];
-#pragma unroll
-    for (int j = k + 1; j < W; ++j) {
-      const double prow = bcast(a[j], p) * inv;
-      a[j] = is_p ? prow : fma(-f, prow, a[j]);
-    }
-    a[k] = is_p ? 1.0 : 0.0;
-    if (is_p) my_k = k;
-  }
-}
-
 // Max over lanes 0..15 of a double, by DPP rotations within the first
 // 16-lane row (lanes 16..63 rotate among themselves and are ignored).
 __device__ __forceinline__ double row16_max(double v) {
@@ -100,20 +60,23 @@ __device__ __forceinline__ double row16_max(double v) {
   return v;
 }
 
-// Gauss-Jordan for NV <= 16 with the same result layout as gauss_jordan.
-// Each lane first scales its row so that its largest |C'| entry is 1 (the
+// Gauss-Jordan on the augmented rows a[0..W) held one per lane (lanes < NV).
+// On return the lane's row is logical row `my_k` of [I | X]; `perm[k]` is
+// the lane holding logical row k (wave-uniform).  Each lane first scales its row so that its largest |C'| entry is 1 (the
 // solution does not change; pivoting becomes scaled partial pivoting).  The
-// pivot is found by one DPP max-reduction of |a[k]| with the lane index
-// packed into the low mantissa bits, the reciprocal of the pivot by
-// v_rcp_f64 and two Newton steps, and every lane -- the pivot lane included
-// -- updates its row with one FMA per column: the pivot lane uses the
-// multiplier 1 - 1/piv, which turns a - (1 - 1/piv) a into a/piv with a
-// relative error of |piv - 1| eps, small because the rows are scaled.
+// pivot is found by a DPP max-reduction of |a[k]| within 16-lane rows (and
+// a max over the rows' results) with the lane index packed into the low
+// mantissa bits, the reciprocal of the pivot by v_rcp_f64 and two Newton
+// steps, and every lane -- the pivot lane included -- updates its row with
+// one FMA per column: the pivot lane uses the multiplier 1 - 1/piv, which
+// turns a - (1 - 1/piv) a into a/piv with a relative error of |piv - 1| eps,
+// small because the rows are scaled.
 template <int NV, int W>
-__device__ __forceinline__ void gauss_jordan16(double (&a)[W], int lane,
-                                               int& my_k, int (&perm)[NV],
-                                               bool& singular) {
-  static_assert(NV <= 16, "one DPP row holds the pivot candidates");
+__device__ __forceinline__ void gauss_jordan_dpp(double (&a)[W], int lane,
+                                                 int& my_k, int (&perm)[NV],
+                                                 bool& singular) {
+  static_assert(NV <= 64, "one wavefront holds the rows");
+  constexpr int kRows = (NV + 15) / 16;
   const bool active = lane < NV;
   {
     double mx = 0.0;
@@ -123,25 +86,30 @@ __device__ __forceinline__ void gauss_jordan16(double (&a)[W], int lane,
 #pragma unroll
     for (int c = 0; c < W; ++c) a[c] *= sc;
   }
-  unsigned used = 0u;
+  unsigned long long used = 0ull;
+  const unsigned long long all =
+      NV == 64 ? ~0ull : ((1ull << NV) - 1ull);
   my_k = -1;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double key = -1.0;
-    if (active && !((used >> lane) & 1u)) {
+    if (active && !((used >> lane) & 1ull)) {
       const double v = fabs(a[k]);
       key = (v == v) ? v : -0.5;  // a NaN row is taken only as a last resort
     }
     long long kb = __double_as_longlong(key);
-    if (key >= 0.0) kb = (kb & ~0xFll) | (15 - lane);
-    const double best = bcast(row16_max(__longlong_as_double(kb)), 0);
+    if (key >= 0.0) kb = (kb & ~0x3Fll) | (63 - lane);
+    const double rm = row16_max(__longlong_as_double(kb));
+    double best = bcast(rm, 0);
+#pragma unroll
+    for (int q = 1; q < kRows; ++q) best = fmax(best, bcast(rm, 16 * q));
     int p;
     if (best >= 0.0)
-      p = 15 - static_cast<int>(__double_as_longlong(best) & 0xF);
+      p = 63 - static_cast<int>(__double_as_longlong(best) & 0x3F);
     else
-      p = __builtin_ctz(~used & ((1u << NV) - 1u));
+      p = __builtin_ctzll(~used & all);
     p = __builtin_amdgcn_readfirstlane(p);
-    used |= 1u << p;
+    used |= 1ull << p;
     perm[k] = p;
     const double piv = bcast(a[k], p);
     singular = singular || !(fabs(piv) > 0.0);
@@ -201,11 +169,7 @@ k_block_thomas(const double* __restrict__ L, const double* __restrict__ C,
         rpre = R[g1 * NV + r];
       }
     }
-    if constexpr (NV <= 16) {
-      gauss_jordan16<NV, W>(a, lane, my_k, perm, singular);
-    } else {
-      gauss_jordan<NV, W>(a, lane, my_k, perm, singular);
-    }
+    gauss_jordan_dpp<NV, W>(a, lane, my_k, perm, singular);
     // store logical row my_k of [C'^-1 U | C'^-1 r']
     if (active) {
       double* wrow = work + (g * NV + my_k) * (NV + 1);

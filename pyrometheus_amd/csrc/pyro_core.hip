@@ -24,7 +24,8 @@ int launch_bt(const double* L, const double* C, const double* U,
 
 // Block sizes with a compiled kernel.
 PYRO_API int pyro_block_thomas_supported(int nv) {
-  return (nv >= 1 && nv <= 16) || nv == 20 || nv == 24 || nv == 32;
+  return (nv >= 1 && nv <= 16) || nv == 20 || nv == 24 || nv == 32 ||
+         nv == 33 || nv == 54;
 }
 
 PYRO_API int pyro_block_thomas_batched(const double* L, const double* C,
@@ -45,7 +46,8 @@ PYRO_API int pyro_block_thomas_batched(const double* L, const double* C,
     PYRO_BT_CASE(5) PYRO_BT_CASE(6) PYRO_BT_CASE(7) PYRO_BT_CASE(8)
     PYRO_BT_CASE(9) PYRO_BT_CASE(10) PYRO_BT_CASE(11) PYRO_BT_CASE(12)
     PYRO_BT_CASE(13) PYRO_BT_CASE(14) PYRO_BT_CASE(15) PYRO_BT_CASE(16)
-    PYRO_BT_CASE(20) PYRO_BT_CASE(24) PYRO_BT_CASE(32)
+    PYRO_BT_CASE(20) PYRO_BT_CASE(24) PYRO_BT_CASE(32) PYRO_BT_CASE(33)
+    PYRO_BT_CASE(54)
     default:
       return pyro::kUnsupported;
   }

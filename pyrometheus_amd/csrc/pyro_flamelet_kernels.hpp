@@ -1,8 +1,9 @@
-// Flamelet and derivative kernels of a generated mechanism (NS <= 16: the
-// NS x NS kinetics Jacobian is held in registers).
+// Flamelet kernels of a generated mechanism that do not depend on how the
+// source Jacobian is formed: residual/block assembly, block Thomas with the
+// scalar Laplacian couplings, update and per-flamelet norms.
 //
-// Included at the end of a generated translation unit after
-// pyro_thermo_kernels.hpp; uses mech::net_production_rates_jac.
+// Included at the end of every generated translation unit after
+// pyro_thermo_kernels.hpp.
 //
 // State layout: phi is (NV, ld) with row 0 the mass enthalpy h and rows
 // 1..NS the mass fractions (pyrometheus/flamelets/state.py:297-316).
@@ -17,130 +18,6 @@ namespace pyro_fl {
 using namespace pyro;
 constexpr int NS = mech::NS;
 constexpr int NV = NS + 1;
-
-// Newton on h(T, Y) = h: take the step, stop once |step| < tol.
-__device__ __forceinline__ bool temperature_from_h(double h,
-                                                   const double (&Y)[NS],
-                                                   double t, double tol,
-                                                   int maxit, double& T) {
-  for (int it = 0; it < maxit; ++it) {
-    double hrt[NS], cpr[NS];
-    mech::species_h_rt(t, 1.0 / t, hrt);
-    mech::species_cp_r(t, cpr);
-    double sh = 0.0, sc = 0.0;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      const double yw = Y[k] * mech::kInvW[k];
-      sh = fma(yw, hrt[k], sh);
-      sc = fma(yw, cpr[k], sc);
-    }
-    const double step = (h - kGasConstant * t * sh) / (kGasConstant * sc);
-    t += step;
-    if (fabs(step) < tol) {
-      T = t;
-      return true;
-    }
-  }
-  T = t;
-  return false;
-}
-
-// Source term S = [vd/rho ; W_k w_k / rho] of one point
-// (flamelets/equations.py:313-389) and, if jout != nullptr, its Jacobian
-// dS/dphi (NV x NV, row-major) by the chain rule through T(h, Y) and
-// rho(T, Y) = p W/(R T):
-//   dT/dh = 1/cp, dT/dY_m = -h_m/cp, drho/dh = -rho/T dT/dh,
-//   drho/dY_m = -rho W/W_m - rho/T dT/dY_m, dC_k = (Y_k/W_k) drho + rho/W_k dY_k.
-__device__ __forceinline__ bool point_source(const double h,
-                                             const double (&Y)[NS],
-                                             const double vd,
-                                             const double tguess,
-                                             const double p, const double tol,
-                                             const int maxit, double& T,
-                                             double (&S)[NV],
-                                             double* __restrict__ jout) {
-  const bool ok = temperature_from_h(h, Y, tguess, tol, maxit, T);
-  double s = 0.0;
-#pragma unroll
-  for (int k = 0; k < NS; ++k) s = fma(mech::kInvW[k], Y[k], s);
-  const double mmw = 1.0 / s;
-  const double rho = p * mmw / (kGasConstant * T);
-  double C[NS], w[NS];
-#pragma unroll
-  for (int k = 0; k < NS; ++k) C[k] = mech::kInvW[k] * rho * Y[k];
-  const double irho = 1.0 / rho;
-  if (jout == nullptr) {
-    mech::net_production_rates(T, C, w);
-  } else {
-    double J[NS][NS], JT[NS];
-    mech::net_production_rates_jac(T, C, w, J, JT);
-    double hrt[NS], cpr[NS];
-    mech::species_h_rt(T, 1.0 / T, hrt);
-    mech::species_cp_r(T, cpr);
-    double sc = 0.0;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) sc = fma(Y[k] * mech::kInvW[k], cpr[k], sc);
-    const double cp = kGasConstant * sc;
-    const double dTdh = 1.0 / cp;
-    const double rT = rho / T;
-    const double drdh = -rT * dTdh;
-    double dTdY[NS], drdY[NS];
-#pragma unroll
-    for (int m = 0; m < NS; ++m) {
-      dTdY[m] = -(kGasConstant * T * hrt[m] * mech::kInvW[m]) * dTdh;
-      drdY[m] = fma(-rT, dTdY[m], -rho * mmw * mech::kInvW[m]);
-    }
-    // row 0: d(vd/rho)
-    const double c0 = -vd * irho * irho;
-    jout[0] = c0 * drdh;
-#pragma unroll
-    for (int m = 0; m < NS; ++m) jout[1 + m] = c0 * drdY[m];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      double a = 0.0;
-#pragma unroll
-      for (int j = 0; j < NS; ++j) a = fma(J[k][j], Y[j] * mech::kInvW[j], a);
-      const double f = mech::kW[k] * irho;
-      const double wr = w[k] * irho;
-      const double dwh = fma(a, drdh, JT[k] * dTdh);
-      double* row = jout + (1 + k) * NV;
-      row[0] = f * fma(-wr, drdh, dwh);
-#pragma unroll
-      for (int m = 0; m < NS; ++m) {
-        const double dwm = fma(J[k][m], rho * mech::kInvW[m],
-                               fma(a, drdY[m], JT[k] * dTdY[m]));
-        row[1 + m] = f * fma(-wr, drdY[m], dwm);
-      }
-    }
-  }
-  S[0] = vd * irho;
-#pragma unroll
-  for (int k = 0; k < NS; ++k) S[1 + k] = mech::kW[k] * w[k] * irho;
-  return ok;
-}
-
-__global__ void __launch_bounds__(256)
-k_flamelet_source(const double* __restrict__ phi, int64_t ldp,
-                  const double* __restrict__ vd,
-                  const double* __restrict__ tguess, int64_t tg_stride,
-                  const double* __restrict__ p, int64_t p_stride,
-                  double* __restrict__ S, int64_t lds,
-                  double* __restrict__ T_out, double* __restrict__ J,
-                  int64_t n, double tol, int maxit, int* __restrict__ n_fail) {
-  const int64_t i = cell_index();
-  if (i >= n) return;
-  double Y[NS], s[NV], T;
-  const double h = phi[i];
-#pragma unroll
-  for (int k = 0; k < NS; ++k) Y[k] = phi[(1 + k) * ldp + i];
-  const bool ok = point_source(h, Y, vd ? vd[i] : 0.0, tguess[i * tg_stride],
-                               p[i * p_stride], tol, maxit, T, s,
-                               J ? J + i * NV * NV : nullptr);
-#pragma unroll
-  for (int v = 0; v < NV; ++v) S[v * lds + i] = s[v];
-  if (T_out) T_out[i] = T;
-  if (!ok && n_fail) atomicAdd(n_fail, 1);
-}
 
 // Residual and block Jacobian of the steady (mode 0) or Crank-Nicolson
 // (mode 1) flamelet equations for a batch of flamelets whose points are
@@ -194,7 +71,7 @@ k_flamelet_assemble(const double* __restrict__ phi, int64_t ldp,
     rhs[i * NV + v] = -r;
   }
   double* blk = J + i * NV * NV;
-#pragma unroll
+#pragma unroll(NV <= 17 ? NV : 1)
   for (int rr = 0; rr < NV; ++rr) {
 #pragma unroll
     for (int cc = 0; cc < NV; ++cc) {
@@ -259,24 +136,8 @@ k_system_norm2(const double* __restrict__ x, const int64_t* __restrict__ off,
 }  // namespace pyro_fl
 
 // ---------------------------------------------------------------------------
-// C ABI (mechanisms with NS <= 16)
+// C ABI
 // ---------------------------------------------------------------------------
-PYRO_API int pyro_flamelet_source(const double* phi, int64_t ldp,
-                                  const double* vd, const double* tguess,
-                                  int64_t tg_stride, const double* p,
-                                  int64_t p_stride, double* S, int64_t lds,
-                                  double* T_out, double* J, int64_t n,
-                                  double tol, int maxit, int* n_fail,
-                                  hipStream_t stream) {
-  if (n == 0) return pyro::kOk;
-  if (n < 0 || ldp < n || lds < n || !phi || !tguess || !p || !S)
-    return pyro::kInvalidArgument;
-  pyro_fl::k_flamelet_source<<<pyro::grid_for(n, 256), 256, 0, stream>>>(
-      phi, ldp, vd, tguess, tg_stride, p, p_stride, S, lds, T_out, J, n, tol,
-      maxit, n_fail);
-  return pyro::launch_status();
-}
-
 PYRO_API int pyro_flamelet_assemble(
     const double* phi, int64_t ldp, const double* S, int64_t lds, double* J,
     const double* chi, const double* la, const double* lb, const double* lc,

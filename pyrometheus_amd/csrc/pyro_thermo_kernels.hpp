@@ -284,85 +284,77 @@ k_wdot_vjp(const double* __restrict__ rho, const double* __restrict__ T,
 // Jacobian of the flamelet source S = [vd/rho; W_k w_k/rho] with respect to
 // phi = [h; Y] at a state whose temperature T is consistent with h, and
 // rho = p W/(R T) (flamelets/equations.py:361-389 differentiated as in
-// :124-129).  Column c is one forward-mode pass along
+// :124-129).  By the chain rule through T(h, Y) and rho(T, Y), column c
+// moves T and rho by
 //   c = 0:      dT = 1/cp,              drho = -rho/T dT
 //   c = 1 + m:  dT = -h_m/cp,           drho = -rho W/W_m - rho/T dT
-// with dC_j = (Y_j/W_j) drho + delta_jm rho/W_m.  Entry (r, c) of cell i
-// goes to out[(r*NV + c)*s_rc + i*s_i]; S itself to S[v*ldS + i] if given.
-__global__ void __launch_bounds__(kBlock)
-k_source_jacobian(const double* __restrict__ T, const double* __restrict__ p,
-                  int64_t p_stride, const double* __restrict__ Y, int64_t ldY,
-                  const double* __restrict__ vd, double* __restrict__ out,
-                  int64_t s_rc, int64_t s_i, double* __restrict__ S,
-                  int64_t ldS, int64_t n) {
-  constexpr int NS = mech::NS, NV = NS + 1;
-  const int64_t i = cell_index();
-  if (i >= n) return;
-  const double t = T[i];
-  double y[NS];
+// and C_j by (Y_j/W_j) drho + delta_jm rho/W_m.  Two passes per chunk of
+// mech::kCacheLd cells: k_reaction_cache stores per reaction k_f, k_r, q,
+// dq/dT, dq/d[M] to a workspace (slot s of chunk cell i at
+// work[s*kCacheLd + i]), and k_source_jacobian_rows assembles each row
+// from the reactions of its species.  Entry (r, c) of cell i goes to
+// out[(r*NV + c)*s_rc + i*s_i].
+__device__ __forceinline__ void source_state(
+    const double* __restrict__ T, const double* __restrict__ p,
+    int64_t p_stride, const double* __restrict__ Y, int64_t ldY, int64_t i,
+    double& t, double& rho, double& mmw, double (&C)[mech::NS]) {
+  t = T[i];
   double sy = 0.0;
 #pragma unroll
-  for (int k = 0; k < NS; ++k) {
-    y[k] = Y[k * ldY + i];
-    sy = fma(y[k], mech::kInvW[k], sy);
-  }
-  const double mmw = 1.0 / sy;
-  const double rho = p[i * p_stride] * mmw / (kGasConstant * t);
-  const double irho = 1.0 / rho;
-  const double v = vd ? vd[i] : 0.0;
-  double C[NS];
+  for (int k = 0; k < mech::NS; ++k) sy = fma(Y[k * ldY + i], mech::kInvW[k], sy);
+  mmw = 1.0 / sy;
+  rho = p[i * p_stride] * mmw / (kGasConstant * t);
 #pragma unroll
-  for (int k = 0; k < NS; ++k) C[k] = mech::kInvW[k] * rho * y[k];
+  for (int k = 0; k < mech::NS; ++k) C[k] = mech::kInvW[k] * rho * Y[k * ldY + i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_reaction_cache(const double* __restrict__ T, const double* __restrict__ p,
+                 int64_t p_stride, const double* __restrict__ Y, int64_t ldY,
+                 double* __restrict__ work, int64_t n) {
+  const int64_t i = cell_index();
+  if (i >= n) return;
+  double t, rho, mmw, C[mech::NS];
+  source_state(T, p, p_stride, Y, ldY, i, t, rho, mmw, C);
+  mech::reaction_cache(t, C, work, static_cast<int>(i));
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_source_jacobian_rows(const double* __restrict__ T,
+                       const double* __restrict__ p, int64_t p_stride,
+                       const double* __restrict__ Y, int64_t ldY,
+                       const double* __restrict__ vd,
+                       const double* __restrict__ work,
+                       double* __restrict__ out, int64_t s_rc, int64_t s_i,
+                       double* __restrict__ S, int64_t ldS, int64_t n) {
+  // dT of column 1+m for every cell of the block, read by every row
+  __shared__ double dTl[mech::NS * kBlock];
+  const int64_t i = cell_index();
+  if (i >= n) return;
+  double t, rho, mmw, C[mech::NS];
+  source_state(T, p, p_stride, Y, ldY, i, t, rho, mmw, C);
+  // one species array live at a time: cp first, then h/RT straight to LDS
   double cp;
   {
-    double cpr[NS];
+    double cpr[mech::NS];
     mech::species_cp_r(t, cpr);
     double sc = 0.0;
 #pragma unroll
-    for (int k = 0; k < NS; ++k) sc = fma(y[k] * mech::kInvW[k], cpr[k], sc);
+    for (int k = 0; k < mech::NS; ++k)
+      sc = fma(Y[k * ldY + i] * mech::kInvW[k], cpr[k], sc);
     cp = kGasConstant * sc;
   }
-  const double rT = rho / t;
-  double* o = out + i * s_i;
-#pragma unroll 1
-  for (int c = 0; c < NV; ++c) {
-    double dT, dr;
-    int m = c - 1;
-    if (c == 0) {
-      dT = 1.0 / cp;
-      dr = -rT * dT;
-    } else {
-      double hrt[NS];
-      mech::species_h_rt(t, 1.0 / t, hrt);
-      double hm = 0.0;
+  {
+    double hrt[mech::NS];
+    mech::species_h_rt(t, 1.0 / t, hrt);
+    const double kT = -kGasConstant * t / cp;
 #pragma unroll
-      for (int k = 0; k < NS; ++k)
-        if (k == m) hm = hrt[k] * mech::kInvW[k];
-      dT = -kGasConstant * t * hm / cp;
-      double iw = 0.0;
-#pragma unroll
-      for (int k = 0; k < NS; ++k)
-        if (k == m) iw = mech::kInvW[k];
-      dr = fma(-rT, dT, -rho * mmw * iw);
-    }
-    double dC[NS], w[NS], dw[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k)
-      dC[k] = fma(y[k] * mech::kInvW[k], dr,
-                  k == m ? rho * mech::kInvW[k] : 0.0);
-    mech::net_production_rates_jvp(t, C, dT, dC, w, dw);
-    o[(0 * NV + c) * s_rc] = -v * irho * irho * dr;
-#pragma unroll
-    for (int k = 0; k < NS; ++k)
-      o[((1 + k) * NV + c) * s_rc] =
-          mech::kW[k] * irho * fma(-w[k] * irho, dr, dw[k]);
-    if (S && c == 0) {
-      S[i] = v * irho;
-#pragma unroll
-      for (int k = 0; k < NS; ++k)
-        S[(1 + k) * ldS + i] = mech::kW[k] * w[k] * irho;
-    }
+    for (int k = 0; k < mech::NS; ++k)
+      dTl[k * kBlock + threadIdx.x] = kT * hrt[k] * mech::kInvW[k];
   }
+  mech::source_jacobian_rows(C, rho, t, cp, mmw, vd ? vd[i] : 0.0, work,
+                             static_cast<int>(i), dTl + threadIdx.x, kBlock,
+                             out + i * s_i, s_rc, S ? S + i : nullptr, ldS);
 }
 
 // ---------------------------------------------------------------------------
@@ -577,18 +569,33 @@ PYRO_API int pyro_wdot_vjp(const double* rho, const double* T, const double* Y,
   return pyro::launch_status();
 }
 
+PYRO_API int64_t pyro_source_jacobian_work_doubles(void) {
+  return static_cast<int64_t>(mech::kCacheSlots) * mech::kCacheLd;
+}
+
 PYRO_API int pyro_source_jacobian(const double* T, const double* p,
                                   int64_t p_stride, const double* Y,
                                   int64_t ldY, const double* vd, double* out,
                                   int64_t s_rc, int64_t s_i, double* S,
-                                  int64_t ldS, int64_t n,
+                                  int64_t ldS, double* work, int64_t n,
                                   hipStream_t stream) {
   using namespace pyro_kern;
   if (n == 0) return pyro::kOk;
   if (n < 0 || ldY < n || !T || !p || !Y || !out || s_rc <= 0 || s_i <= 0 ||
-      (S && ldS < n))
+      (S && ldS < n) || !work)
     return pyro::kInvalidArgument;
-  k_source_jacobian<<<pyro::grid_for(n, kBlock), kBlock, 0, stream>>>(
-      T, p, p_stride, Y, ldY, vd, out, s_rc, s_i, S, ldS, n);
-  return pyro::launch_status();
+  constexpr int64_t chunk = mech::kCacheLd;
+  for (int64_t c0 = 0; c0 < n; c0 += chunk) {
+    const int64_t nc = n - c0 < chunk ? n - c0 : chunk;
+    k_reaction_cache<<<pyro::grid_for(nc, kBlock), kBlock, 0, stream>>>(
+        T + c0, p + c0 * p_stride, p_stride, Y + c0, ldY, work, nc);
+    k_source_jacobian_rows<<<pyro::grid_for(nc, kBlock), kBlock, 0,
+                             stream>>>(
+        T + c0, p + c0 * p_stride, p_stride, Y + c0, ldY,
+        vd ? vd + c0 : nullptr, work, out + c0 * s_i, s_rc, s_i,
+        S ? S + c0 : nullptr, ldS, nc);
+    const int st = pyro::launch_status();
+    if (st != pyro::kOk) return st;
+  }
+  return pyro::kOk;
 }

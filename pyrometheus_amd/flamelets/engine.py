@@ -8,7 +8,9 @@ launch sequence:
 
 1. ``pyro_flamelet_source`` -- per point: T(h, Y) by Newton, rho, S and the
    (nv x nv) source Jacobian (chain rule through the analytic kinetics
-   Jacobian);
+   Jacobian held in registers; mechanisms with at most 16 species), or, for
+   larger mechanisms, ``pyro_temperature`` for T(h, Y) followed by
+   ``pyro_source_jacobian`` (per-reaction cache and row assembly);
 2. ``pyro_flamelet_assemble`` -- residual and block-tridiagonal Jacobian,
    steady or Crank-Nicolson per flamelet;
 3. ``pyro_flamelet_thomas`` -- block Thomas, one wave per flamelet, with the
@@ -51,12 +53,22 @@ def _rows(t, name, writable=False):
 
 class FlameletBatch:
     def __init__(self, gas, grids: Sequence[np.ndarray], y_ox, y_fu,
-                 device=None):
+                 device=None, jacobian: str = "auto"):
         lib = gas._lib
-        if not lib.has("pyro_flamelet_source"):
-            raise HipError(f"mechanism {gas._mech.name} has no flamelet "
-                           "kernels (more than 16 species)")
+        if not lib.has("pyro_flamelet_assemble"):
+            raise HipError(f"library of {gas._mech.name} has no flamelet "
+                           "kernels")
         self.gas, self.lib = gas, lib
+        # one-pass register Jacobian (small mechanisms) or the two-pass
+        # cached form (any mechanism); "auto" takes the first available
+        if jacobian not in ("auto", "dense", "cached"):
+            raise ValueError(f"unknown jacobian form {jacobian!r}")
+        if jacobian == "dense" and not lib.has("pyro_flamelet_source"):
+            raise HipError(f"{gas._mech.name}: no register-Jacobian kernel "
+                           "(more than 16 species)")
+        self.dense = (jacobian != "cached"
+                      and lib.has("pyro_flamelet_source"))
+        self._jac_work = None
         self.dev = torch.device(device) if device is not None else gas.device
         self.ns = gas.num_species
         self.nv = self.ns + 1
@@ -146,11 +158,44 @@ class FlameletBatch:
         phi = _rows(phi, "phi")
         vd, tguess, p = (_rows(x, "input") for x in (vd, tguess, p))
         self.n_fail.zero_()
-        self._call("pyro_flamelet_source", phi.data_ptr(), phi.stride(0),
-                      _ptr(vd), tguess.data_ptr(), 1, p.data_ptr(), 1,
-                      self.S.data_ptr(), self.P, self.T.data_ptr(),
-                      self.J.data_ptr() if jac else None, self.P, tol, maxit,
-                      self.n_fail.data_ptr(), _stream())
+        if self.dense:
+            self._call("pyro_flamelet_source", phi.data_ptr(), phi.stride(0),
+                       _ptr(vd), tguess.data_ptr(), 1, p.data_ptr(), 1,
+                       self.S.data_ptr(), self.P, self.T.data_ptr(),
+                       self.J.data_ptr() if jac else None, self.P, tol, maxit,
+                       self.n_fail.data_ptr(), _stream())
+            return
+        # T(h, Y): take the Newton step, stop once |step| < tol, as the
+        # one-pass kernel does
+        P, ldp = self.P, phi.stride(0)
+        y_ptr = phi.data_ptr() + 8 * ldp
+        self._call("pyro_temperature", phi.data_ptr(), 1, tguess.data_ptr(),
+                   1, y_ptr, ldp, self.T.data_ptr(), P, 0, 0, tol, maxit,
+                   self.n_fail.data_ptr(), None, _stream())
+        if jac:
+            if self._jac_work is None:
+                need = int(self.lib.call("pyro_source_jacobian_work_doubles"))
+                self._jac_work = torch.empty(need, dtype=torch.float64,
+                                             device=self.dev)
+            nv = self.nv
+            self._call("pyro_source_jacobian", self.T.data_ptr(),
+                       p.data_ptr(), 1, y_ptr, ldp, _ptr(vd),
+                       self.J.data_ptr(), 1, nv * nv, self.S.data_ptr(), P,
+                       self._jac_work.data_ptr(), P, _stream())
+        else:
+            # S alone: rho = p W/(R T), then W_k w_k / rho
+            rho = torch.empty(P, dtype=torch.float64, device=self.dev)
+            self._call("pyro_mixture_state", 2, p.data_ptr(), 1,
+                       self.T.data_ptr(), 1, y_ptr, ldp, rho.data_ptr(), P,
+                       _stream())
+            w = self.S[1:]
+            self._call("pyro_wdot", rho.data_ptr(), self.T.data_ptr(), y_ptr,
+                       ldp, w.data_ptr(), self.P, P, _stream())
+            w.mul_(self.gas.molecular_weights[:, None] / rho)
+            if vd is None:
+                self.S[0].zero_()
+            else:
+                torch.div(vd, rho, out=self.S[0])
 
     def assemble(self, phi, chi, modes: Optional[torch.Tensor] = None,
                  phi_prev=None, r_prev=None, r_out=None):

@@ -352,7 +352,8 @@ def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
                 equilibrate_steps=0, equilibrate_dt0=1e-8,
                 equilibrate_growth=2.5, newton_norm="reference",
                 kernel_events: Optional[dict] = None,
-                stats: Optional[dict] = None) -> List[FlameletResult]:
+                stats: Optional[dict] = None,
+                jacobian: str = "auto") -> List[FlameletResult]:
     """Solve every problem with the control flow of ``FlameletSolver.solve``
     applied to each flamelet independently (preceded, when
     ``equilibrate_steps`` > 0, by ``FlameletSolver.equilibrate``); one launch
@@ -360,7 +361,8 @@ def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
     (``FlameletStepper``).  ``newton_norm`` is the update norm of
     ``update_norm``; ``kernel_events``, if a dict, collects HIP events around
     every kernel (``FlameletBatch.kernel_ms``); ``stats``, if a dict,
-    receives the iteration count and the sum of batch points over them."""
+    receives the iteration count and the sum of batch points over them;
+    ``jacobian`` picks the source-Jacobian kernels (``FlameletBatch``)."""
     from .stepper import FlameletStepper, SolveSettings
     settings = SolveSettings(
         newton_maxiter=newton_maxiter, newton_tol=newton_tol,
@@ -368,7 +370,8 @@ def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
         bdf_time_step=bdf_time_step, bdf_maxsteps=bdf_maxsteps,
         try_newton=try_newton, max_attempts=max_attempts,
         equilibrate_steps=equilibrate_steps, equilibrate_dt0=equilibrate_dt0,
-        equilibrate_growth=equilibrate_growth, newton_norm=newton_norm)
+        equilibrate_growth=equilibrate_growth, newton_norm=newton_norm,
+        jacobian=jacobian)
     st = FlameletStepper(pyro_gas, settings, verbosity=verbosity,
                          kernel_events=kernel_events)
     handles = st.add(problems)

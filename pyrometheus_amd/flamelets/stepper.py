@@ -45,6 +45,7 @@ class SolveSettings:
     equilibrate_dt0: float = 1e-8
     equilibrate_growth: float = 2.5
     newton_norm: str = "reference"
+    jacobian: str = "auto"  # FlameletBatch source-Jacobian form
 
 
 # per-flamelet integer/float control state, in this order
@@ -166,7 +167,8 @@ class FlameletStepper:
         probs = [sl.problem for sl in self._slots]
         b = FlameletBatch(self.gas, [p.x for p in probs],
                           np.stack([p.y_ox for p in probs]),
-                          np.stack([p.y_fu for p in probs]), device=self.dev)
+                          np.stack([p.y_fu for p in probs]), device=self.dev,
+                          jacobian=self.s.jacobian)
         h_ox = np.array([p.h_ox for p in probs], dtype=float)
         h_fu = np.array([p.h_fu for p in probs], dtype=float)
         b.set_boundary_enthalpy(h_ox, h_fu)

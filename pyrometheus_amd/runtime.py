@@ -177,7 +177,8 @@ MECH_SIGNATURES = {
                       _I64, _P),
     "pyro_wdot_vjp": (_P, _P, _P, _I64, _P, _I64, _P, _P, _P, _I64, _I64, _P),
     "pyro_source_jacobian": (_P, _P, _I64, _P, _I64, _P, _P, _I64, _I64, _P,
-                             _I64, _I64, _P),
+                             _I64, _P, _I64, _P),
+    "pyro_source_jacobian_work_doubles": (),
     "pyro_flamelet_source": (_P, _I64, _P, _P, _I64, _P, _I64, _P, _I64, _P,
                              _P, _I64, _D, _I, _P, _P),
     "pyro_flamelet_assemble": (_P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P,
@@ -200,7 +201,9 @@ class HipLibrary:
             if fn is None:
                 continue
             fn.argtypes = list(argtypes)
-            fn.restype = ctypes.c_int
+            # size queries return int64; everything else an int status
+            fn.restype = (ctypes.c_int64 if name.endswith("_doubles")
+                          else ctypes.c_int)
             self.functions[name] = fn
 
     def has(self, name: str) -> bool:

@@ -124,9 +124,9 @@ class ChemistryServer:
         if req.mechanism not in self.gases:
             raise KeyError(f"mechanism {req.mechanism!r} is not loaded")
         if isinstance(req, FlameletRequest) and not self.gases[
-                req.mechanism]._lib.has("pyro_flamelet_source"):
-            raise ValueError(f"{req.mechanism}: flamelet kernels are built "
-                             "for mechanisms with at most 16 species")
+                req.mechanism]._lib.has("pyro_flamelet_assemble"):
+            raise ValueError(f"{req.mechanism}: library has no flamelet "
+                             "kernels")
         kind = "wdot" if isinstance(req, WdotRequest) else "flamelet"
         nbytes = self.cost(req)
         if nbytes > self.budget:
@@ -295,7 +295,7 @@ def poisson_workload(gases: Dict[str, object], n_requests: int, rate: float,
     names = list(gases)
     wm = list(wdot_mechanisms or names)
     fm = list(flamelet_mechanisms or [n for n in names if gases[n]._lib.has(
-        "pyro_flamelet_source")])
+        "pyro_flamelet_assemble")])
     t = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
     t -= t[0]
     out = []

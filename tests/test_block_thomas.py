@@ -56,7 +56,7 @@ def test_oracle_scalar_thomas():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nv", [1, 2, 5, 9, 10, 16, 20])
+@pytest.mark.parametrize("nv", [1, 2, 5, 9, 10, 16, 20, 54])
 @pytest.mark.parametrize("diag", [False, True])
 def test_gpu_batched_ragged(nv, diag, device):
     import torch

@@ -33,7 +33,8 @@ def test_alone_equals_in_mix_and_budget(gases, device):
     from pyrometheus_amd.flamelets.solver import solve_batch
     arrivals = poisson_workload(gases, 14, rate=40.0, seed=9,
                                 wdot_log2_cells=(10, 14),
-                                flamelet_points=(40, 160))
+                                flamelet_points=(40, 160),
+                                flamelet_mechanisms=["sandiego"])
     kinds = [type(r).__name__ for _, r in arrivals]
     print("[serve] mix", kinds, flush=True)
     assert "WdotRequest" in kinds and "FlameletRequest" in kinds
