@@ -60,8 +60,8 @@ def parse():
     ap.add_argument("--flamelet-reps", type=int, default=3)
     ap.add_argument("--flamelet-seed", type=int, default=4)
     ap.add_argument("--no-flamelets", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=48)
-    ap.add_argument("--serve-rate", type=float, default=16.0)
+    ap.add_argument("--serve-requests", type=int, default=96)
+    ap.add_argument("--serve-rate", type=float, default=48.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-wdot", action="store_true",
                     help="skip the omega-dot leg (profiling the flamelets)")

@@ -895,7 +895,7 @@ def _emit_reaction_cache(w: _Writer, L: _Lowering):
     w("constexpr int kCacheLd = 65536;")
     w("// per-reaction k_f, k_r, q, dq/dT, dq/d[M] to the workspace")
     w("__device__ __forceinline__ void reaction_cache(const double T, "
-      "const double (&C)[NS], double* __restrict__ work, const int il) {")
+      "const double (&C)[NS], double* __restrict__ work, const unsigned il) {")
     w.ind += 1
     _deriv_prologue(w, L)
     for i, r in enumerate(m.reactions):
@@ -904,7 +904,7 @@ def _emit_reaction_cache(w: _Writer, L: _Lowering):
         L.kf_block_deriv(w, r)
         _reverse_parts(w, L, r)
         def at(nm):
-            return f"work[{slots[(i, nm)]} * kCacheLd + il]"
+            return f"(work + {slots[(i, nm)]} * kCacheLd)[il]"
         w(f"{at('kf')} = kf;")
         if r.reversible:
             w(f"{at('kr')} = kf * kinv;")
@@ -938,7 +938,7 @@ def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
     w("__device__ __forceinline__ void source_jacobian_rows("
       "double (&C)[NS], const double rho, const double T, "
       "const double cp, const double mmw, const double vd, "
-      "const double* __restrict__ work, const int il, "
+      "const double* __restrict__ work, const unsigned il, "
       "const double* __restrict__ dTl, const int hs, "
       "double* __restrict__ o, const int64_t s_rc, "
       "double* __restrict__ S, const int64_t ldS) {")
@@ -966,7 +966,7 @@ def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
     for i, r in enumerate(m.reactions):
         for k, nu in L.species_updates(r, "q"):
             inv[k].append((i, nu))
-    for k in range(ns):
+    for k in range(L.tuning.get("jac_rows", ns)):
         w(f"{{  // row {k + 1}: {m.species[k].name}")
         w.ind += 1
         # fresh copies of C and of the per-cell scalars for this row:
@@ -987,7 +987,7 @@ def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
             w(f"{{  // R{i}: {r.equation}")
             w.ind += 1
             def at(nm, i=i):
-                return f"work[{slots[(i, nm)]} * kCacheLd + il]"
+                return f"(work + {slots[(i, nm)]} * kCacheLd)[il]"
             w(f"const double kf = {at('kf')};")
             w(f"wk = fma({nl}, {at('q')}, wk);")
             w(f"bk = fma({nl}, {at('dqT')}, bk);")

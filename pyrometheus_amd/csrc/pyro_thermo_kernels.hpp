@@ -316,7 +316,7 @@ k_reaction_cache(const double* __restrict__ T, const double* __restrict__ p,
   if (i >= n) return;
   double t, rho, mmw, C[mech::NS];
   source_state(T, p, p_stride, Y, ldY, i, t, rho, mmw, C);
-  mech::reaction_cache(t, C, work, static_cast<int>(i));
+  mech::reaction_cache(t, C, work, static_cast<unsigned>(i));
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -353,7 +353,7 @@ k_source_jacobian_rows(const double* __restrict__ T,
       dTl[k * kBlock + threadIdx.x] = kT * hrt[k] * mech::kInvW[k];
   }
   mech::source_jacobian_rows(C, rho, t, cp, mmw, vd ? vd[i] : 0.0, work,
-                             static_cast<int>(i), dTl + threadIdx.x, kBlock,
+                             static_cast<unsigned>(i), dTl + threadIdx.x, kBlock,
                              out + i * s_i, s_rc, S ? S + i : nullptr, ldS);
 }
 
