@@ -159,7 +159,7 @@ k_block_thomas(const double* __restrict__ L, const double* __restrict__ C,
     const int64_t g = off + i;
     // block row i+1 of C and r, row = lane, loaded before the elimination
     // so that its latency overlaps it (the flamelet form)
-    double cpre[NV], rpre = 0.0;
+    double cpre[NV], rpre = 0.0, lpre = 0.0, upre = 0.0;
     if constexpr (DIAG) {
       if (i + 1 < n) {
         const int64_t g1 = g + 1;
@@ -167,6 +167,8 @@ k_block_thomas(const double* __restrict__ L, const double* __restrict__ C,
 #pragma unroll
         for (int c = 0; c < NV; ++c) cpre[c] = C[(g1 * NV + r) * NV + c];
         rpre = R[g1 * NV + r];
+        lpre = L[g];
+        if (i + 2 < n) upre = U[g1];
       }
     }
     gauss_jordan_dpp<NV, W>(a, lane, my_k, perm, singular);
@@ -186,7 +188,7 @@ k_block_thomas(const double* __restrict__ L, const double* __restrict__ C,
     if constexpr (DIAG) {
       // this lane continues with logical row k: take row k of the
       // prefetched block from the lane that loaded it
-      const double l = L[g];
+      const double l = lpre;
 #pragma unroll
       for (int c = 0; c < NV; ++c)
         nxt[c] = fma(-l, a[NV + c], __shfl(cpre[c], k));
@@ -209,7 +211,7 @@ k_block_thomas(const double* __restrict__ L, const double* __restrict__ C,
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
       if constexpr (DIAG) {
-        nxt[NV + c] = (has_u && c == k) ? U[g1] : 0.0;
+        nxt[NV + c] = (has_u && c == k) ? upre : 0.0;
       } else {
         nxt[NV + c] = has_u ? U[(g1 * NV + k) * NV + c] : 0.0;
       }
