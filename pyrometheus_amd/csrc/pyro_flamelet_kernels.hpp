@@ -12,6 +12,7 @@
 #pragma once
 
 #include "pyro_block_thomas.hpp"
+#include "pyro_spike.hpp"
 
 namespace pyro_fl {
 
@@ -185,4 +186,37 @@ PYRO_API int pyro_flamelet_update(double* phi, int64_t ldp, const double* x,
   pyro_fl::k_flamelet_update<<<pyro::grid_for(n, 256), 256, 0, stream>>>(
       phi, ldp, x, sys, active, n);
   return pyro::launch_status();
+}
+
+// Partitioned block Thomas for the flamelet blocks (nv <= 32): the same
+// system as pyro_flamelet_thomas, with each system cut into partitions
+// [pstart[p], pend[p]) of system psys[p] (consecutive within a system) and
+// the interfaces between consecutive partitions solved as a block-
+// tridiagonal system with 2nv x 2nv blocks (roff: interface offsets per
+// system; jpart[j]: left partition of interface j; jbefore/jafter[p]:
+// interface on either side of partition p or -1).  Work arrays: fact
+// P*nv*(2nv+1), maps nparts*2*nv*(2nv+1), Lr/Cr/Ur nj*(2nv)^2, Rr and Z
+// nj*2nv, rwork nj*2nv*(2nv+1) doubles.
+PYRO_API int pyro_flamelet_thomas_partitioned(
+    const double* L, const double* C, const double* U, const double* R,
+    double* X, const int64_t* off, int64_t nsys, const int64_t* pstart,
+    const int64_t* pend, const int* psys, int64_t nparts, const int* jpart,
+    const int64_t* jbefore, const int64_t* jafter, const int64_t* roff,
+    int64_t nj, double* fact, double* maps, double* Lr, double* Cr,
+    double* Ur, double* Rr, double* Z, double* rwork, int* n_singular,
+    hipStream_t stream) {
+  if (nsys == 0 || nparts == 0) return pyro::kOk;
+  if constexpr (pyro_fl::NV > 32) {
+    return pyro::kUnsupported;
+  } else {
+    if (nsys < 0 || nparts < 0 || nj < 0 || !L || !C || !U || !R || !X ||
+        !off || !pstart || !pend || !psys || !jbefore || !jafter || !fact ||
+        !maps || (nj > 0 && (!jpart || !roff || !Lr || !Cr || !Ur || !Rr ||
+                             !Z || !rwork)))
+      return pyro::kInvalidArgument;
+    return pyro_spike::solve<pyro_fl::NV>(
+        L, C, U, R, X, off, pstart, pend, psys, nparts, jpart, jbefore,
+        jafter, roff, nsys, nj, fact, maps, Lr, Cr, Ur, Rr, Z, rwork,
+        n_singular, stream);
+  }
 }

@@ -1,0 +1,269 @@
+// Partitioned block Thomas (a SPIKE-type solver) for block-tridiagonal
+// systems whose off-diagonal blocks are scalar multiples of the identity
+// (the flamelet Laplacian couplings):
+//   L[i-1] x_{i-1} + C_i x_i + U[i] x_{i+1} = r_i
+// (L[g] couples row g+1 to x_g, U[g] couples row g to x_{g+1}, as in
+// k_block_thomas<NV, true>).  The result is that of the block Thomas
+// algorithm of pyrometheus/flamelets/linear_solver.py:6-103; only the order
+// of elimination changes.
+//
+// Every system is cut into partitions [s, e) of a few dozen rows.  Each
+// partition q is reduced by two wavefronts at once:
+//   down: rows s..e-1 with the coupling to x_{s-1} carried as NV extra
+//         right-hand-side columns; gives x_{e-1} = a + B x_{s-1} + D x_e and
+//         stores [X_U | X_r | X_W] of every row for the recovery,
+//   up:   rows e-1..s with the coupling to x_e carried the same way; gives
+//         x_s = a' + B' x_{s-1} + D' x_e.
+// With b_q = x_{e-1} of partition q and t_q = x_s of partition q, the
+// interface unknowns Z_j = (b_q, t_{q+1}) form a block-tridiagonal system
+// with 2NV x 2NV blocks, solved by k_block_thomas<2NV, false>; then every
+// partition recovers its rows from the stored factors:
+//   x_i = X_r,i + X_W,i x_{s-1} - X_U,i x_{i+1}.
+// The chain each wavefront walks is the partition length plus the number
+// of partitions instead of the system length.
+#pragma once
+
+#include "pyro_block_thomas.hpp"
+
+namespace pyro_spike {
+
+using pyro_bt::bcast;
+
+// One sweep over a partition, one wavefront, lane = logical row of
+// [C' | slot | r' | E] (W = 3 NV + 1 doubles): slot holds the coupling to
+// the next row to be eliminated (the neighbour outside the partition at
+// the last one), E the coefficients of the outside unknown met first.
+template <int NV>
+__global__ void __launch_bounds__(64)
+k_spike_sweep(const double* __restrict__ L, const double* __restrict__ C,
+              const double* __restrict__ U, const double* __restrict__ R,
+              const int64_t* __restrict__ off,
+              const int64_t* __restrict__ pstart,
+              const int64_t* __restrict__ pend,
+              const int* __restrict__ psys, double* __restrict__ fact,
+              double* __restrict__ maps, int* __restrict__ n_singular) {
+  constexpr int W = 3 * NV + 1;
+  constexpr int MW = 2 * NV + 1;  // [a | B | D] per logical row
+  const int part = blockIdx.x >> 1;
+  const bool down = (blockIdx.x & 1) == 0;
+  const int lane = threadIdx.x;
+  const bool active = lane < NV;
+  const int r = active ? lane : 0;
+  const int64_t s = pstart[part], e = pend[part];
+  const int sy = psys[part];
+  const int64_t s0 = off[sy], s1 = off[sy + 1];
+  const int64_t m = e - s;
+  // the up map of a system's first partition is never used
+  if (!down && s == s0) return;
+  bool singular = false;
+  // row index of step t, its coupling to the next row of the sweep (the
+  // "slot"), and the next row's coupling back to it
+  auto row = [&](int64_t t) { return down ? s + t : e - 1 - t; };
+  auto slot_coef = [&](int64_t i) -> double {
+    if (down) return i < s1 - 1 ? U[i] : 0.0;
+    return i > s0 ? L[i - 1] : 0.0;
+  };
+  auto back_coef = [&](int64_t i) -> double {  // next row's coupling to row i
+    return down ? L[i] : U[i - 1];
+  };
+  double a[W];
+  int perm[NV];
+  int my_k = lane;
+  {
+    const int64_t i = row(0);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a[c] = C[(i * NV + r) * NV + c];
+    const double sc = slot_coef(i);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a[NV + c] = (c == r) ? sc : 0.0;
+    a[2 * NV] = R[i * NV + r];
+    // the outside unknown met first: x_{s-1} (down) or x_e (up)
+    double ec = 0.0;
+    if (down && s > s0) ec = -L[s - 1];
+    if (!down && e < s1) ec = -U[e - 1];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a[2 * NV + 1 + c] = (c == r) ? ec : 0.0;
+  }
+  for (int64_t t = 0; t < m; ++t) {
+    const int64_t i = row(t);
+    double cpre[NV], rpre = 0.0, bpre = 0.0, spre = 0.0;
+    if (t + 1 < m) {
+      const int64_t i1 = row(t + 1);
+#pragma unroll
+      for (int c = 0; c < NV; ++c) cpre[c] = C[(i1 * NV + r) * NV + c];
+      rpre = R[i1 * NV + r];
+      bpre = back_coef(i);
+      spre = slot_coef(i1);
+    }
+    pyro_bt::gauss_jordan_dpp<NV, W>(a, lane, my_k, perm, singular);
+    if (down && active) {
+      // [X_U | X_r | X_W] of logical row my_k
+      double* f = fact + (i * NV + my_k) * MW;
+#pragma unroll
+      for (int c = 0; c < MW; ++c) f[c] = a[NV + c];
+    }
+    if (t + 1 == m) break;
+    const int k = active ? my_k : 0;
+    const double l = bpre;
+#pragma unroll
+    for (int c = 0; c < NV; ++c)
+      a[c] = fma(-l, a[NV + c], __shfl(cpre[c], k));
+    a[2 * NV] = fma(-l, a[2 * NV], __shfl(rpre, k));
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a[2 * NV + 1 + c] *= -l;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a[NV + c] = (c == k) ? spre : 0.0;
+  }
+  // map of the last row of the sweep: x = X_r + X_E x_first - X_slot x_next
+  //   down: x_{e-1} = X_r + X_W x_{s-1} - X_U x_e
+  //   up:   x_s     = X_r + X_V x_e     - X_L x_{s-1}
+  // stored as [a | B (coeff of x_{s-1}) | D (coeff of x_e)]
+  if (active) {
+    double* mp = maps + ((static_cast<int64_t>(part) * 2 + (down ? 0 : 1)) *
+                             NV + my_k) * MW;
+    mp[0] = a[2 * NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      const double ext = a[2 * NV + 1 + c], nxt = -a[NV + c];
+      mp[1 + c] = down ? ext : nxt;
+      mp[1 + NV + c] = down ? nxt : ext;
+    }
+  }
+  if (singular && lane == 0 && n_singular) atomicAdd(n_singular, 1);
+}
+
+// Interface system: row j = interface between partitions q and q+1 of a
+// system, unknown Z_j = (b_q, t_{q+1}):
+//   b_q     - D_q t_{q+1} - B_q b_{q-1}          = a_q       (down map of q)
+//   t_{q+1} - B'_{q+1} b_q - D'_{q+1} t_{q+2}     = a'_{q+1}  (up map of q+1)
+// Blocks in k_block_thomas<2NV, false> layout: Lr[j-1] couples row j to
+// Z_{j-1} = (b_{q-1}, t_q), Ur[j] couples row j to Z_{j+1} = (b_{q+1},
+// t_{q+2}).  jpart[j] = global index of partition q.
+template <int NV>
+__global__ void __launch_bounds__(64)
+k_spike_interfaces(const double* __restrict__ maps,
+                   const int* __restrict__ jpart, int64_t nj,
+                   double* __restrict__ Lr, double* __restrict__ Cr,
+                   double* __restrict__ Ur, double* __restrict__ Rr) {
+  constexpr int N2 = 2 * NV;
+  constexpr int MW = 2 * NV + 1;
+  const int64_t j = blockIdx.x;
+  if (j >= nj) return;
+  const int q = jpart[j];
+  const double* dn = maps + (static_cast<int64_t>(q) * 2 + 0) * NV * MW;
+  const double* up = maps + (static_cast<int64_t>(q + 1) * 2 + 1) * NV * MW;
+  for (int e = threadIdx.x; e < N2 * N2; e += 64) {
+    const int rr = e / N2, cc = e % N2;
+    double cv, lv = 0.0, uv = 0.0;
+    if (rr < NV) {
+      const double* m = dn + rr * MW;
+      if (cc < NV) {
+        cv = (rr == cc) ? 1.0 : 0.0;
+        lv = -m[1 + cc];               // -B_q on b_{q-1}
+      } else {
+        cv = -m[1 + NV + (cc - NV)];   // -D_q on t_{q+1}
+      }
+    } else {
+      const double* m = up + (rr - NV) * MW;
+      if (cc < NV) {
+        cv = -m[1 + cc];               // -B'_{q+1} on b_q
+      } else {
+        cv = (rr == cc) ? 1.0 : 0.0;
+        uv = -m[1 + NV + (cc - NV)];   // -D'_{q+1} on t_{q+2}
+      }
+    }
+    Cr[j * N2 * N2 + e] = cv;
+    Ur[j * N2 * N2 + e] = uv;
+    // Lr[j-1] holds row j's coupling to Z_{j-1}; the first interface of a
+    // system has none (its B_q is zero)
+    if (j > 0) Lr[(j - 1) * N2 * N2 + e] = lv;
+  }
+  if (threadIdx.x < N2) {
+    const int rr = threadIdx.x;
+    Rr[j * N2 + rr] = rr < NV ? dn[rr * MW] : up[(rr - NV) * MW];
+  }
+}
+
+// Recovery: x_i = X_r,i + X_W,i x_{s-1} - X_U,i x_{i+1}, from the last row
+// of the partition up, with x_{s-1} = b_{q-1} and x_e = t_{q+1} taken from
+// the interface solution Z (jbefore/jafter: interface index or -1).
+template <int NV>
+__global__ void __launch_bounds__(64)
+k_spike_recover(const double* __restrict__ fact,
+                const int64_t* __restrict__ pstart,
+                const int64_t* __restrict__ pend,
+                const int64_t* __restrict__ jbefore,
+                const int64_t* __restrict__ jafter,
+                const double* __restrict__ Z, double* __restrict__ X) {
+  constexpr int MW = 2 * NV + 1;
+  constexpr int N2 = 2 * NV;
+  const int part = blockIdx.x;
+  const int lane = threadIdx.x;
+  const bool active = lane < NV;
+  const int r = active ? lane : 0;
+  const int64_t s = pstart[part], e = pend[part];
+  const int64_t jb = jbefore[part], ja = jafter[part];
+  double xs[NV], xe[NV];
+#pragma unroll
+  for (int m = 0; m < NV; ++m) {
+    xs[m] = jb >= 0 ? Z[jb * N2 + m] : 0.0;        // b_{q-1}
+    xe[m] = ja >= 0 ? Z[ja * N2 + NV + m] : 0.0;   // t_{q+1}
+  }
+  double xn[NV];
+#pragma unroll
+  for (int m = 0; m < NV; ++m) xn[m] = xe[m];
+  double fc[MW];
+  {
+    const double* f = fact + ((e - 1) * NV + r) * MW;
+#pragma unroll
+    for (int c = 0; c < MW; ++c) fc[c] = f[c];
+  }
+  for (int64_t i = e - 1; i >= s; --i) {
+    double fnext[MW];
+    if (i > s) {
+      const double* f = fact + ((i - 1) * NV + r) * MW;
+#pragma unroll
+      for (int c = 0; c < MW; ++c) fnext[c] = f[c];
+    }
+    double xr = fc[NV];
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+      xr = fma(fc[NV + 1 + m], xs[m], xr);
+      xr = fma(-fc[m], xn[m], xr);
+    }
+    if (active) X[i * NV + r] = xr;
+#pragma unroll
+    for (int m = 0; m < NV; ++m) xn[m] = bcast(xr, m);
+    if (i > s) {
+#pragma unroll
+      for (int c = 0; c < MW; ++c) fc[c] = fnext[c];
+    }
+  }
+}
+
+// Host side: the four launches.  All arrays are device arrays prepared by
+// the caller (see pyro_flamelet_thomas_partitioned).
+template <int NV>
+int solve(const double* L, const double* C, const double* U, const double* R,
+          double* X, const int64_t* off, const int64_t* pstart,
+          const int64_t* pend, const int* psys, int64_t nparts,
+          const int* jpart, const int64_t* jbefore, const int64_t* jafter,
+          const int64_t* roff, int64_t nsys, int64_t nj, double* fact,
+          double* maps, double* Lr, double* Cr, double* Ur, double* Rr,
+          double* Z, double* rwork, int* n_singular, hipStream_t stream) {
+  k_spike_sweep<NV><<<static_cast<unsigned>(2 * nparts), 64, 0, stream>>>(
+      L, C, U, R, off, pstart, pend, psys, fact, maps, n_singular);
+  if (nj > 0) {
+    k_spike_interfaces<NV><<<static_cast<unsigned>(nj), 64, 0, stream>>>(
+        maps, jpart, nj, Lr, Cr, Ur, Rr);
+    pyro_bt::k_block_thomas<2 * NV, false>
+        <<<static_cast<unsigned>(nsys), 64, 0, stream>>>(Lr, Cr, Ur, Rr, Z,
+                                                         roff, rwork,
+                                                         n_singular);
+  }
+  k_spike_recover<NV><<<static_cast<unsigned>(nparts), 64, 0, stream>>>(
+      fact, pstart, pend, jbefore, jafter, Z, X);
+  return pyro::launch_status();
+}
+
+}  // namespace pyro_spike

@@ -13,8 +13,11 @@ launch sequence:
    ``pyro_source_jacobian`` (per-reaction cache and row assembly);
 2. ``pyro_flamelet_assemble`` -- residual and block-tridiagonal Jacobian,
    steady or Crank-Nicolson per flamelet;
-3. ``pyro_flamelet_thomas`` -- block Thomas, one wave per flamelet, with the
-   Laplacian coupling blocks as scalar multiples of the identity;
+3. ``pyro_flamelet_thomas_partitioned`` -- block Thomas with the Laplacian
+   coupling blocks as scalar multiples of the identity, each flamelet cut
+   into partitions swept by two wavefronts each and joined through a small
+   interface system (``pyro_flamelet_thomas``, one wavefront per flamelet,
+   for nv > 32 or when asked);
 4. ``pyro_flamelet_update`` -- phi += v for the selected flamelets and the
    per-flamelet sums of squares of v over the enthalpy row and over the
    species rows (fixed-order reductions, so a flamelet's result does not
@@ -40,6 +43,36 @@ def _ptr(t):
     return None if t is None else t.data_ptr()
 
 
+def partition_table(sizes, part_len=None):
+    """Cut systems of the given sizes into consecutive partitions of about
+    ``part_len`` rows (default sqrt(2 N) per system, at least 8): returns
+    numpy arrays pstart, pend, psys, jpart, jbefore, jafter, roff (see
+    ``pyro_flamelet_thomas_partitioned``)."""
+    import math
+    pstart, pend, psys, jpart, jbefore, jafter = [], [], [], [], [], []
+    roff = [0]
+    o = 0
+    for s, n in enumerate(sizes):
+        m = part_len or max(8, int(round(math.sqrt(2.0 * n))))
+        npart = max(1, -(-n // m))
+        first = len(pstart)
+        nj0 = roff[-1]
+        for q in range(npart):
+            pstart.append(o + (q * n) // npart)
+            pend.append(o + ((q + 1) * n) // npart)
+            psys.append(s)
+            jbefore.append(nj0 + q - 1 if q > 0 else -1)
+            jafter.append(nj0 + q if q < npart - 1 else -1)
+        for q in range(npart - 1):
+            jpart.append(first + q)
+        roff.append(nj0 + npart - 1)
+        o += n
+    i64 = lambda a: np.asarray(a, dtype=np.int64)
+    i32 = lambda a: np.asarray(a, dtype=np.int32)
+    return (i64(pstart), i64(pend), i32(psys), i32(jpart), i64(jbefore),
+            i64(jafter), i64(roff))
+
+
 def _rows(t, name, writable=False):
     """A contiguous (rows, points) array, the layout the kernels read with
     one leading dimension for all state arrays; read-only inputs in another
@@ -53,7 +86,8 @@ def _rows(t, name, writable=False):
 
 class FlameletBatch:
     def __init__(self, gas, grids: Sequence[np.ndarray], y_ox, y_fu,
-                 device=None, jacobian: str = "auto"):
+                 device=None, jacobian: str = "auto",
+                 thomas: str = "auto", part_len: Optional[int] = None):
         lib = gas._lib
         if not lib.has("pyro_flamelet_assemble"):
             raise HipError(f"library of {gas._mech.name} has no flamelet "
@@ -111,6 +145,30 @@ class FlameletBatch:
                                        device=self.dev)
         # kernel name -> list of (start, end) HIP events when timing is on
         self.events = None
+        # partitioned block Thomas (nv <= 32)
+        self.partitioned = (lib.has("pyro_flamelet_thomas_partitioned")
+                            and self.nv <= 32 and thomas != "sequential")
+        if self.partitioned:
+            self._setup_partitions(part_len)
+
+    def _setup_partitions(self, part_len=None):
+        tabs = partition_table(self.sizes, part_len)
+        t = lambda a: torch.as_tensor(a, device=self.dev)
+        (self.p_start, self.p_end, self.p_sys, self.j_part, self.j_before,
+         self.j_after, self.r_off) = (t(a) for a in tabs)
+        self.n_parts = len(tabs[0])
+        self.n_j = len(tabs[3])
+        nv, n2 = self.nv, 2 * self.nv
+        dd = dict(dtype=torch.float64, device=self.dev)
+        self.fact = torch.empty(self.P * nv * (2 * nv + 1), **dd)
+        self.maps = torch.empty(self.n_parts * 2 * nv * (2 * nv + 1), **dd)
+        nj = max(self.n_j, 1)
+        self.Lr = torch.empty(nj * n2 * n2, **dd)
+        self.Cr = torch.empty(nj * n2 * n2, **dd)
+        self.Ur = torch.empty(nj * n2 * n2, **dd)
+        self.Rr = torch.empty(nj * n2, **dd)
+        self.Z = torch.empty(nj * n2, **dd)
+        self.rwork = torch.empty(nj * n2 * (n2 + 1), **dd)
 
     def time_kernels(self, on=True):
         """Record HIP events around every kernel launch (``kernel_ms``)."""
@@ -213,6 +271,21 @@ class FlameletBatch:
                       self.U.data_ptr(), _ptr(r_out), self.P, _stream())
 
     def thomas(self):
+        if self.partitioned:
+            self._call("pyro_flamelet_thomas_partitioned", self.L.data_ptr(),
+                       self.J.data_ptr(), self.U.data_ptr(),
+                       self.rhs.data_ptr(), self.x.data_ptr(),
+                       self.off.data_ptr(), self.B, self.p_start.data_ptr(),
+                       self.p_end.data_ptr(), self.p_sys.data_ptr(),
+                       self.n_parts, self.j_part.data_ptr(),
+                       self.j_before.data_ptr(), self.j_after.data_ptr(),
+                       self.r_off.data_ptr(), self.n_j,
+                       self.fact.data_ptr(), self.maps.data_ptr(),
+                       self.Lr.data_ptr(), self.Cr.data_ptr(),
+                       self.Ur.data_ptr(), self.Rr.data_ptr(),
+                       self.Z.data_ptr(), self.rwork.data_ptr(), None,
+                       _stream())
+            return
         self._call("pyro_flamelet_thomas", self.L.data_ptr(),
                       self.J.data_ptr(), self.U.data_ptr(),
                       self.rhs.data_ptr(), self.x.data_ptr(),

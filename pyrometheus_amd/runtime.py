@@ -185,6 +185,10 @@ MECH_SIGNATURES = {
                                _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64,
                                _P),
     "pyro_flamelet_thomas": (_P, _P, _P, _P, _P, _P, _I64, _P, _P, _P),
+    "pyro_flamelet_thomas_partitioned": (_P, _P, _P, _P, _P, _P, _I64, _P,
+                                         _P, _P, _I64, _P, _P, _P, _P, _I64,
+                                         _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                         _P),
     "pyro_flamelet_update": (_P, _I64, _P, _P, _P, _P, _I64, _P, _I64, _P),
 }
 

@@ -142,3 +142,49 @@ def test_gpu_badly_scaled_rows(diag, device):
         tol = 1e-12
     err = np.max(np.abs(x - ref)) / np.max(np.abs(ref))
     assert err < tol, err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("part_len", [None, 2, 3, 8])
+def test_gpu_partitioned_matches_oracle(part_len, device):
+    """The partitioned flamelet block Thomas (partitions swept from both
+    ends, interface system, recovery) against the oracle on ragged systems
+    with scalar couplings, and against the one-wavefront kernel."""
+    import torch
+    from pyrometheus_amd import bundled, get_thermochem_class
+    from pyrometheus_amd.flamelets.engine import FlameletBatch
+    gas = get_thermochem_class(bundled("sandiego"))(device=device)
+    nv = 10
+    sizes = [3, 4, 17, 64, 250, 1000, 5]
+    rng = np.random.default_rng(19)
+    grids = [np.linspace(0, 1, n) for n in sizes]
+    y = np.zeros(nv - 1)
+    y[-1] = 1.0
+    bp = FlameletBatch(gas, grids, y, y, part_len=part_len)
+    bs = FlameletBatch(gas, grids, y, y, thomas="sequential")
+    assert bp.partitioned and not bs.partitioned
+    P = sum(sizes)
+    Lb, Ub = np.zeros(P), np.zeros(P)
+    Cb, Rb = np.zeros((P, nv, nv)), np.zeros((P, nv))
+    systems, o = [], 0
+    for j, n in enumerate(sizes):
+        _, C, _, r = _system(rng, n, nv, pivot_stress=(j % 2 == 1))
+        lv, uv = rng.normal(size=n - 1), rng.normal(size=n - 1)
+        Lb[o:o + n - 1], Ub[o:o + n - 1] = lv, uv
+        Cb[o:o + n], Rb[o:o + n] = C, r
+        systems.append((lv[:, None, None] * np.eye(nv), C,
+                        uv[:, None, None] * np.eye(nv), r, o, n))
+        o += n
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=device)
+    for b in (bp, bs):
+        b.L.copy_(t(Lb))
+        b.U.copy_(t(Ub))
+        b.J.copy_(t(Cb))
+        b.rhs.copy_(t(Rb))
+        b.thomas()
+    xp, xs = bp.x.cpu().numpy(), bs.x.cpu().numpy()
+    for L, C, U, r, o, n in systems:
+        ref = block_thomas(L, C, U, r)
+        sc = np.max(np.abs(ref))
+        assert np.max(np.abs(xp[o:o + n] - ref)) <= 1e-12 * sc, n
+        assert np.max(np.abs(xp[o:o + n] - xs[o:o + n])) <= 1e-12 * sc, n
