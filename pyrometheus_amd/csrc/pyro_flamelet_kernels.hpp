@@ -34,13 +34,16 @@ __global__ void __launch_bounds__(256)
 k_flamelet_assemble(const double* __restrict__ phi, int64_t ldp,
                     const double* __restrict__ S, int64_t lds,
                     double* __restrict__ J, const double* __restrict__ chi,
+                    const double* __restrict__ chi0,
                     const double* __restrict__ la,
                     const double* __restrict__ lb,
                     const double* __restrict__ lc,
                     const int* __restrict__ sys,
                     const int64_t* __restrict__ off,
                     const double* __restrict__ bc,  // (nsys, 2, NV)
-                    const int* __restrict__ modes,  // (nsys,): 0 steady, 1 CN
+                    // (nsys,): bit 0 Crank-Nicolson (else steady), bit 1
+                    // the dissipation rate chi0 (else chi)
+                    const int* __restrict__ modes,
                     const double* __restrict__ phi_prev,
                     const double* __restrict__ r_prev,
                     const double* __restrict__ dt,  // (nsys,)
@@ -52,10 +55,12 @@ k_flamelet_assemble(const double* __restrict__ phi, int64_t ldp,
   const int s = sys[i];
   const bool first = (i == off[s]);
   const bool last = (i == off[s + 1] - 1);
-  const double hc = 0.5 * chi[i];
+  const int mode0 = modes ? modes[s] : 0;
+  const double* __restrict__ chs = ((mode0 & 2) && chi0) ? chi0 : chi;
+  const double hc = 0.5 * chs[i];
   const double ai = la[i], bi = lb[i], ci = lc[i];
   const double mask = (first || last) ? 0.0 : 1.0;
-  const int mode = modes ? modes[s] : 0;
+  const int mode = mode0 & 1;
   const double hdt = (mode == 1) ? 0.5 * dt[s] : 0.0;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -82,22 +87,27 @@ k_flamelet_assemble(const double* __restrict__ phi, int64_t ldp,
       blk[rr * NV + cc] = val;
     }
   }
-  const double lo = last ? 0.0 : 0.5 * chi[i + 1] * la[i + 1];
+  const double lo = last ? 0.0 : 0.5 * chs[i + 1] * la[i + 1];
   const double up = last ? 0.0 : hc * ci;
   L[i] = (mode == 1) ? -hdt * lo : lo;
   U[i] = (mode == 1) ? -hdt * up : up;
 }
 
-// phi += x for points of active systems; x is (P, NV) AoS.
+// phi += x for points of active systems (x is (P, NV) AoS) and, if tg is
+// given, tg = T_new for the same points.
 __global__ void __launch_bounds__(256)
 k_flamelet_update(double* __restrict__ phi, int64_t ldp,
                   const double* __restrict__ x, const int* __restrict__ sys,
-                  const int* __restrict__ active, int64_t n) {
+                  const int* __restrict__ active,
+                  const double* __restrict__ T_new, double* __restrict__ tg,
+                  int64_t n) {
   const int64_t i = cell_index();
   if (i >= n) return;
   if (active && !active[sys[i]]) return;
 #pragma unroll
   for (int v = 0; v < NV; ++v) phi[v * ldp + i] += x[i * NV + v];
+  // the next temperature guess of an updated point is this update's T
+  if (tg) tg[i] = T_new[i];
 }
 
 // Deterministic per-system sums of squares of x (P, NV), split into the
@@ -141,7 +151,8 @@ k_system_norm2(const double* __restrict__ x, const int64_t* __restrict__ off,
 // ---------------------------------------------------------------------------
 PYRO_API int pyro_flamelet_assemble(
     const double* phi, int64_t ldp, const double* S, int64_t lds, double* J,
-    const double* chi, const double* la, const double* lb, const double* lc,
+    const double* chi, const double* chi0, const double* la,
+    const double* lb, const double* lc,
     const int* sys, const int64_t* off, const double* bc, const int* modes,
     const double* phi_prev, const double* r_prev, const double* dt,
     double* rhs, double* L, double* U, double* r_out, int64_t n,
@@ -152,8 +163,8 @@ PYRO_API int pyro_flamelet_assemble(
       (modes && (!phi_prev || !r_prev || !dt)))
     return pyro::kInvalidArgument;
   pyro_fl::k_flamelet_assemble<<<pyro::grid_for(n, 256), 256, 0, stream>>>(
-      phi, ldp, S, lds, J, chi, la, lb, lc, sys, off, bc, modes, phi_prev,
-      r_prev, dt, rhs, L, U, r_out, n);
+      phi, ldp, S, lds, J, chi, chi0, la, lb, lc, sys, off, bc, modes,
+      phi_prev, r_prev, dt, rhs, L, U, r_out, n);
   return pyro::launch_status();
 }
 
@@ -173,18 +184,20 @@ PYRO_API int pyro_flamelet_thomas(const double* L, const double* C,
 
 PYRO_API int pyro_flamelet_update(double* phi, int64_t ldp, const double* x,
                                   const int* sys, const int* active,
+                                  const double* T_new, double* tg,
                                   const int64_t* off, int64_t nsys,
                                   double* norm2, int64_t n,
                                   hipStream_t stream) {
   if (n == 0) return pyro::kOk;
-  if (n < 0 || ldp < n || !phi || !x || !sys) return pyro::kInvalidArgument;
+  if (n < 0 || ldp < n || !phi || !x || !sys || (tg && !T_new))
+    return pyro::kInvalidArgument;
   if (norm2) {
     if (!off || nsys <= 0) return pyro::kInvalidArgument;
     pyro_fl::k_system_norm2<<<static_cast<unsigned>(nsys), 256, 0, stream>>>(
         x, off, norm2);
   }
   pyro_fl::k_flamelet_update<<<pyro::grid_for(n, 256), 256, 0, stream>>>(
-      phi, ldp, x, sys, active, n);
+      phi, ldp, x, sys, active, T_new, tg, n);
   return pyro::launch_status();
 }
 

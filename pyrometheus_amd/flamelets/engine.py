@@ -256,13 +256,18 @@ class FlameletBatch:
                 torch.div(vd, rho, out=self.S[0])
 
     def assemble(self, phi, chi, modes: Optional[torch.Tensor] = None,
-                 phi_prev=None, r_prev=None, r_out=None):
-        phi, chi, phi_prev, r_prev = (_rows(x, "input") for x in
-                                      (phi, chi, phi_prev, r_prev))
+                 phi_prev=None, r_prev=None, r_out=None, chi0=None):
+        """Residual and Jacobian blocks.  ``modes`` (B,) int32: bit 0 Crank-
+        Nicolson with ``phi_prev``, ``r_prev`` and ``self.dt``, bit 1 the
+        dissipation rate ``chi0`` in place of ``chi``."""
+        phi, chi, phi_prev, r_prev, chi0 = (_rows(x, "input") for x in
+                                            (phi, chi, phi_prev, r_prev,
+                                             chi0))
         r_out = _rows(r_out, "r_out", writable=True)
         self._call("pyro_flamelet_assemble", phi.data_ptr(), phi.stride(0),
                       self.S.data_ptr(), self.P, self.J.data_ptr(),
-                      chi.data_ptr(), self.la.data_ptr(), self.lb.data_ptr(),
+                      chi.data_ptr(), _ptr(chi0), self.la.data_ptr(),
+                      self.lb.data_ptr(),
                       self.lc.data_ptr(), self.sys.data_ptr(),
                       self.off.data_ptr(), self.bc.data_ptr(), _ptr(modes),
                       _ptr(phi_prev), _ptr(r_prev),
@@ -292,28 +297,41 @@ class FlameletBatch:
                       self.off.data_ptr(), self.B, self.work.data_ptr(), None,
                       _stream())
 
-    def update(self, phi, active: Optional[torch.Tensor]):
+    def update(self, phi, active: Optional[torch.Tensor], tg=None):
+        """phi += v on the flamelets ``active`` selects and the per-flamelet
+        norms of v; with ``tg`` also tg = T on the same points (the next
+        temperature guess)."""
         phi = _rows(phi, "phi", writable=True)
+        tg = _rows(tg, "tg", writable=True)
         self._call("pyro_flamelet_update", phi.data_ptr(), phi.stride(0),
                       self.x.data_ptr(), self.sys.data_ptr(), _ptr(active),
-                      self.off.data_ptr(), self.B, self.norm2.data_ptr(),
-                      self.P, _stream())
+                      self.T.data_ptr() if tg is not None else None,
+                      _ptr(tg), self.off.data_ptr(), self.B,
+                      self.norm2.data_ptr(), self.P, _stream())
 
-    def residual(self, phi, chi, vd, tguess, p, out):
-        """Steady residual R(phi) into ``out`` (nv, P).  The block storage is
-        overwritten with values that are not used."""
+    def residual(self, phi, chi, vd, tguess, p, out, modes=None,
+                 phi_prev=None, r_prev=None, chi0=None):
+        """Steady residual R(phi) into ``out`` (nv, P), with the dissipation
+        rate chosen per flamelet by bit 1 of ``modes`` as in ``assemble``
+        (bit 0 changes only the outputs that are not used here).  The block
+        storage is overwritten with values that are not used."""
         self.source(phi, vd, tguess, p, jac=False)
-        self.assemble(phi, chi, r_out=out)
+        self.assemble(phi, chi, modes, phi_prev, r_prev, r_out=out,
+                      chi0=chi0)
         return out
 
     def newton_update(self, phi, chi, vd, tguess, p, modes=None,
-                      phi_prev=None, r_prev=None, apply=True, active=None):
+                      phi_prev=None, r_prev=None, apply=True, active=None,
+                      chi0=None, refresh_guess=False):
         """One Newton update v = -J^-1 R for every flamelet (steady, or
-        Crank-Nicolson where ``modes`` is 1).  Applies it when ``apply``;
-        returns the per-flamelet sums of squares of v, (B, 2) on the device
-        as [enthalpy row, species rows], and v (P, nv)."""
+        Crank-Nicolson and/or with ``chi0`` per the bits of ``modes``).
+        Applies it when ``apply`` (and then, with ``refresh_guess``, sets
+        ``tguess`` to the temperature of the updated points); returns the
+        per-flamelet sums of squares of v, (B, 2) on the device as
+        [enthalpy row, species rows], and v (P, nv)."""
         self.source(phi, vd, tguess, p, jac=True)
-        self.assemble(phi, chi, modes, phi_prev, r_prev)
+        self.assemble(phi, chi, modes, phi_prev, r_prev, chi0=chi0)
         self.thomas()
-        self.update(phi, active if apply else self.none_active)
+        self.update(phi, active if apply else self.none_active,
+                    tguess if (apply and refresh_guess) else None)
         return self.norm2, self.x

@@ -172,7 +172,6 @@ class FlameletStepper:
         h_ox = np.array([p.h_ox for p in probs], dtype=float)
         h_fu = np.array([p.h_fu for p in probs], dtype=float)
         b.set_boundary_enthalpy(h_ox, h_fu)
-        b.dt.fill_(float(self.s.bdf_time_step))
         if self.kernel_events is not None:
             b.events = self.kernel_events
         from .solver import enthalpy_scale
@@ -187,6 +186,16 @@ class FlameletStepper:
                               dtype=float if k == "err_prev" else np.int64)
                   for k in _CTRL}
         self.r_tmp = torch.empty_like(self.A["phi"])
+        # control block [modes | active | dt] of 4B int32 words, pinned on
+        # the host; the previous upload from it has finished when step()
+        # returns (step() reads the norms back before it returns)
+        pinned = self.dev.type == "cuda"
+        self._ctl_pin = torch.zeros(4 * b.B, dtype=torch.int32,
+                                    pin_memory=pinned)
+        self._ctl_np = self._ctl_pin.numpy()
+        self._ctl_dev = torch.zeros(4 * b.B, dtype=torch.int32,
+                                    device=self.dev)
+        b.dt = self._ctl_dev[2 * b.B:].view(torch.float64)
         self.sys = b.sys.long()
         for sl in self._slots:  # the packed arrays now own the state
             sl.arrays = {}
@@ -211,24 +220,29 @@ class FlameletStepper:
                                         A["chi0"], A["vd"], A["pres"])
         live = (phase != DONE_OK) & (phase != DONE_FAIL)
         eq = phase == EQUIL
-        chi_eff = torch.where(self._pts(eq), chi0, chi) if eq.any() else chi
+        cn = (phase == MARCH) | eq
         dts = np.where(eq, s.equilibrate_dt0 * s.equilibrate_growth
                        ** C["eq_step"], s.bdf_time_step)
-        b.dt.copy_(torch.as_tensor(dts, dtype=torch.float64))
-        cn = (phase == MARCH) | eq
+        # the per-flamelet controls travel in one upload: modes (bit 0
+        # Crank-Nicolson, bit 1 the equilibration dissipation rate), the
+        # active flags and the time steps (b.dt is a view of the upload)
+        nb, host = b.B, self._ctl_np
+        host[:nb] = cn.astype(np.int32) | (eq.astype(np.int32) << 1)
+        host[nb:2 * nb] = live
+        host[2 * nb:4 * nb].view(np.float64)[:] = dts
+        self._ctl_dev.copy_(self._ctl_pin, non_blocking=True)
+        modes, active = self._ctl_dev[:nb], self._ctl_dev[nb:2 * nb]
         starting = cn & (C["inner"] == 0)
         if starting.any():
-            b.residual(phi, chi_eff, vd, tg, pres, self.r_tmp)
+            b.residual(phi, chi, vd, tg, pres, self.r_tmp, modes,
+                       A["phi_prev"], A["r_prev"], chi0)
             pm = self._pts(starting)
             A["phi_prev"][:, pm] = phi[:, pm]
             A["r_prev"][:, pm] = self.r_tmp[:, pm]
-        modes = torch.as_tensor(cn.astype(np.int32), device=self.dev)
-        active = torch.as_tensor(live.astype(np.int32), device=self.dev)
-        norm2, _ = b.newton_update(phi, chi_eff, vd, tg, pres, modes,
+        norm2, _ = b.newton_update(phi, chi, vd, tg, pres, modes,
                                    A["phi_prev"], A["r_prev"], apply=True,
-                                   active=active)
-        # refresh the temperature guess of live flamelets only
-        tg.copy_(torch.where(self._pts(live), b.T, tg))
+                                   active=active, chi0=chi0,
+                                   refresh_guess=True)
         delta = update_norm(norm2.cpu().numpy(), s.newton_norm, self.h_scale)
         self.iterations += 1
         self.point_iterations += b.P

@@ -182,14 +182,15 @@ MECH_SIGNATURES = {
     "pyro_flamelet_source": (_P, _I64, _P, _P, _I64, _P, _I64, _P, _I64, _P,
                              _P, _I64, _D, _I, _P, _P),
     "pyro_flamelet_assemble": (_P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P,
-                               _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64,
-                               _P),
+                               _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                               _I64, _P),
     "pyro_flamelet_thomas": (_P, _P, _P, _P, _P, _P, _I64, _P, _P, _P),
     "pyro_flamelet_thomas_partitioned": (_P, _P, _P, _P, _P, _P, _I64, _P,
                                          _P, _P, _I64, _P, _P, _P, _P, _I64,
                                          _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                          _P),
-    "pyro_flamelet_update": (_P, _I64, _P, _P, _P, _P, _I64, _P, _I64, _P),
+    "pyro_flamelet_update": (_P, _I64, _P, _P, _P, _P, _P, _P, _I64, _P,
+                             _I64, _P),
 }
 
 
