@@ -100,7 +100,19 @@ class _Writer:
 #:  kc    -- "species": K_c^-1 as a product of per-species exp(g'_k) factors
 #:           (one exp per species instead of one per reaction); "exp": one
 #:           exp of -ln K_c per reversible reaction;
-DEFAULT_TUNING = {"pin": True, "pin_every": 4, "kc": "species", "ctable": True}
+#:  lazy  -- the omega-dot and JVP kernels open species (load C_k, compute
+#:           exp(g'_k)) at their first use and store each rate after its last
+#:           update, along a reaction order that keeps few species open
+#:           (``reaction_order``); ``order_tries`` greedy restarts;
+#:  wdot_waves / deriv_waves -- waves per SIMD the compiler must budget
+#:           registers for in k_wdot / k_wdot_jvp (0: its own choice);
+#:  lazy_fence -- an empty asm with a memory clobber every that many
+#:           reactions of the lazy passes.
+#: (GRI-3.0, 2^24 cells: 16.6 ms with the species arrays held in registers,
+#: 12.2 ms lazy at two waves per SIMD.)
+DEFAULT_TUNING = {"pin": True, "pin_every": 2, "kc": "species", "ctable": True,
+                  "lazy": True, "wdot_waves": 2, "deriv_waves": 0,
+                  "lazy_fence": 1, "order_tries": 8}
 
 
 def element_shift(mech: Mechanism):
@@ -156,6 +168,8 @@ class _Lowering:
         self.species_kc = (self.tuning["kc"] == "species" and all(
             balanced_integer(r) for r in mech.reactions if r.reversible))
         self.tab = None  # list of tabulated constants while emitting a table
+        # exp of the Arrhenius and Gibbs exponents (bounded arguments)
+        self.fexp = "exp"
 
     _INLINE = {0.0, 0.5, -0.5, 1.0, -1.0, 2.0, -2.0, 4.0, -4.0}
 
@@ -244,33 +258,41 @@ class _Lowering:
     def emit_gibbs_exp(self, w: _Writer):
         """E[k] = exp(g'_k): both NASA ranges in the basis
         (1 - ln T, T, T^2, T^3, T^4, 1/T, 1), one select per species."""
+        self.emit_gibbs_prologue(w)
+        w("double E[NS];")
+        for k in range(self.ns):
+            self.emit_gibbs_one(w, k)
+
+    def emit_gibbs_prologue(self, w: _Writer):
         w("const double omlnT = 1.0 - lnT;")
         w("const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;")
         self._flags(w)
-        w("double E[NS];")
-        for k, sp in enumerate(self.m.species):
-            def coeffs(c):
-                return [c[0], -c[1] / 2, -c[2] / 6, -c[3] / 12, -c[4] / 20,
-                        c[5] - self.shift_m[k], -(c[6] + self.shift_n[k])]
-            basis = ["omlnT", "T", "T2", "T3", "T4", "invT"]
-
-            def poly(cs):
-                expr = self.c(cs[6])
-                for cc, b in zip(reversed(cs[:6]), reversed(basis)):
-                    expr = f"fma({self.c(cc)}, {b}, {expr})"
-                return expr
-            hi, lo = coeffs(sp.high), coeffs(sp.low)
-            if hi == lo:
-                w(f"E[{k}] = exp({poly(hi)});  // {sp.name}")
-            else:
-                # both ranges evaluated and pinned so the choice stays a
-                # per-lane select rather than a divergent branch
-                w(f"{{ double gh = {poly(hi)}, gl = {poly(lo)};")
-                w('  asm volatile("" : "+v"(gh), "+v"(gl));')
-                w(f"  E[{k}] = exp({self.tmid_flag[sp.t_mid]} ? gh : gl); }}"
-                  f"  // {sp.name}")
         w("const double rtp = kGasConstant * T / kOneAtm;")
         w("const double prt = kOneAtm / (kGasConstant * T);")
+
+    def emit_gibbs_one(self, w: _Writer, k: int):
+        sp = self.m.species[k]
+
+        def coeffs(c):
+            return [c[0], -c[1] / 2, -c[2] / 6, -c[3] / 12, -c[4] / 20,
+                    c[5] - self.shift_m[k], -(c[6] + self.shift_n[k])]
+        basis = ["omlnT", "T", "T2", "T3", "T4", "invT"]
+
+        def poly(cs):
+            expr = self.c(cs[6])
+            for cc, b in zip(reversed(cs[:6]), reversed(basis)):
+                expr = f"fma({self.c(cc)}, {b}, {expr})"
+            return expr
+        hi, lo = coeffs(sp.high), coeffs(sp.low)
+        if hi == lo:
+            w(f"E[{k}] = {self.fexp}({poly(hi)});  // {sp.name}")
+        else:
+            # both ranges evaluated and pinned so the choice stays a
+            # per-lane select rather than a divergent branch
+            w(f"{{ double gh = {poly(hi)}, gl = {poly(lo)};")
+            w('  asm volatile("" : "+v"(gh), "+v"(gl));')
+            w(f"  E[{k}] = {self.fexp}({self.tmid_flag[sp.t_mid]} ? gh : gl);"
+              f" }}  // {sp.name}")
 
     def kinv_expr(self, r: Reaction) -> str:
         """exp(-ln K_c) from the E[] factors (balanced, integer nu)."""
@@ -303,11 +325,12 @@ class _Lowering:
                 if b < 0:
                     return f"({c(a)} * pyro::ipow<{int(-b)}>(invT))"
                 return f"({c(a)} * pyro::ipow<{int(b)}>(T))"
-            return f"({c(a)} * exp({c(b)} * lnT))"
+            return f"({c(a)} * {self.fexp}({c(b)} * lnT))"
         lna = math.log(a)
         if b == 0:
-            return f"exp(fma({c(-ta)}, invT, {c(lna)}))"
-        return f"exp(fma({c(b)}, lnT, fma({c(-ta)}, invT, {c(lna)})))"
+            return f"{self.fexp}(fma({c(-ta)}, invT, {c(lna)}))"
+        return (f"{self.fexp}(fma({c(b)}, lnT, fma({c(-ta)}, invT, "
+                f"{c(lna)})))")
 
     def rate_ratio(self, lo, hi) -> str:
         """k_lo / k_hi as one expression: a constant times an integer power
@@ -323,8 +346,8 @@ class _Lowering:
             return f"({c} * pyro::ipow<{int(db)}>(T))"
         dlna = math.log(lo.A) - math.log(hi.A)
         dta = lo.t_a - hi.t_a
-        return (f"exp(fma({self.c(db)}, lnT, fma({self.c(-dta)}, invT, "
-                f"{self.c(dlna)})))")
+        return (f"{self.fexp}(fma({self.c(db)}, lnT, fma({self.c(-dta)}, "
+                f"invT, {self.c(dlna)})))")
 
     def third_body(self, r: Reaction, C: str = "C", Ctot: str = "Ctot") -> str:
         """[M] = d Ctot + sum_k (eff_k - d) C_k (chem_expr.py:327-345); with
@@ -528,6 +551,8 @@ def _emit_prelude(w: _Writer, L: _Lowering, name: str, digest: str,
     w(f"constexpr int NF = {m.num_falloff};")
     w(f"constexpr int kAbiVersion = {ABI_VERSION};")
     w(f"constexpr int kWdotBlock = {wdot_block};")
+    w(f"constexpr int kWdotWaves = {int(L.tuning.get('wdot_waves', 0))};")
+    w(f"constexpr int kDerivWaves = {int(L.tuning.get('deriv_waves', 0))};")
     w("__device__ constexpr double kW[NS] = {"
       + ", ".join(lit(x) for x in m.molecular_weights) + "};")
     w("__device__ constexpr double kInvW[NS] = {"
@@ -629,6 +654,209 @@ def _emit_wdot(w: _Writer, L: _Lowering):
     if L.tab is not None:
         tab = L.tab or [0.0]
         decl = ("__constant__ double kTabWdot[] = {"
+                + ", ".join(lit(x) for x in tab) + "};")
+        w.lines.insert(table_at, decl)
+        L.tab = None
+
+
+def species_uses(L: _Lowering, r: Reaction):
+    """(concentrations read, Gibbs factors read, rates written) by reaction
+    ``r`` in the fused kernel, as sets of species indices."""
+    c = {L.idx[sp] for sp, _ in r.reactants}
+    e, w = set(), {k for k, _ in L.species_updates(r, "q")}
+    if r.reversible:
+        c |= {L.idx[sp] for sp, _ in r.products}
+        if L.species_kc:
+            e = {L.idx[sp] for sp, _ in list(r.reactants) + list(r.products)}
+    if r.kind in ("three-body", "falloff"):
+        c |= {k for k, _ in L.efficiency_terms(r)}
+    return c, e, w
+
+
+def _peak_live(uses, order) -> int:
+    """Largest number of species values (concentrations, Gibbs factors,
+    rates) open at once along ``order``."""
+    first, last = {}, {}
+    for pos, ri in enumerate(order):
+        for which, ks in enumerate(uses[ri]):
+            for k in ks:
+                first.setdefault((which, k), pos)
+                last[(which, k)] = pos
+    delta = [0] * (len(order) + 1)
+    for key, a in first.items():
+        delta[a] += 1
+        delta[last[key] + 1] -= 1
+    peak = run = 0
+    for d in delta:
+        run += d
+        peak = max(peak, run)
+    return peak
+
+
+def _greedy_order(uses, rank) -> List[int]:
+    left: Dict[Tuple[int, int], int] = {}
+    for u in uses:
+        for which, ks in enumerate(u):
+            for k in ks:
+                left[(which, k)] = left.get((which, k), 0) + 1
+    opened = set()
+    remaining = list(range(len(uses)))
+    order = []
+    while remaining:
+        best, best_key = None, None
+        for ri in remaining:
+            score = 0
+            for which, ks in enumerate(uses[ri]):
+                for k in ks:
+                    score += (which, k) not in opened
+                    score -= left[(which, k)] == 1
+            key = (score, rank[ri])
+            if best_key is None or key < best_key:
+                best, best_key = ri, key
+        remaining.remove(best)
+        order.append(best)
+        for which, ks in enumerate(uses[best]):
+            for k in ks:
+                opened.add((which, k))
+                left[(which, k)] -= 1
+    return order
+
+
+def reaction_order(L: _Lowering, tries: int = 8) -> List[int]:
+    """An order of the reactions that keeps few species live at once: a
+    species' concentration and Gibbs factor are opened by the first reaction
+    that reads them and its rate is closed (stored) by the last that writes
+    it.  Greedy: take the reaction that opens the fewest new values net of
+    the values it closes, ties broken by a fixed ranking of the reactions
+    (their index, then seeded permutations); the order with the smallest
+    peak is kept.  The net production rates are sums over reactions, so the
+    order changes only their rounding."""
+    import random
+    uses = [species_uses(L, r) for r in L.m.reactions]
+    n = len(uses)
+    best, best_peak = None, None
+    for t in range(tries):
+        rank = list(range(n))
+        if t:
+            random.Random(t).shuffle(rank)
+        order = _greedy_order(uses, rank)
+        peak = _peak_live(uses, order)
+        if best_peak is None or peak < best_peak:
+            best, best_peak = order, peak
+    return best
+
+
+def _emit_wdot_lazy(w: _Writer, L: _Lowering):
+    """Net production rates of one cell read from and written to memory
+    directly: reactions in ``reaction_order``, each species' concentration
+    loaded (and Gibbs factor computed) just before its first use and its
+    rate stored right after its last update, so that only the species of a
+    sliding window occupy registers."""
+    m = L.m
+    w("// net production rates, species opened and closed along the "
+      "reactions (codegen/python.py:475-495)")
+    sig = ("__device__ __forceinline__ void net_production_rates_lazy("
+           "const double rho, const double T, const double* __restrict__ Y, "
+           "const int64_t ldY, const int64_t i, double* __restrict__ W, "
+           "const int64_t ldW) {")
+    rev = any(r.reversible for r in m.reactions)
+    if not L.tuning.get("lazy", True) or (rev and not L.species_kc):
+        w(sig)
+        w.ind += 1
+        w("double C[NS], w[NS];")
+        w("#pragma unroll")
+        w("for (int k = 0; k < NS; ++k) C[k] = kInvW[k] * rho * Y[k * ldY + i];")
+        w("net_production_rates(T, C, w);")
+        w("#pragma unroll")
+        w("for (int k = 0; k < NS; ++k) W[k * ldW + i] = w[k];")
+        w.ind -= 1
+        w("}")
+        w("")
+        return
+    table_at = len(w.lines)
+    w(sig)
+    w.ind += 1
+    if L.tuning.get("ctable", True):
+        L.tab = []
+        w("pyro::cptr K = (pyro::cptr)kTabWdotLazy;")
+        w('asm volatile("" : "+s"(K));')
+    w("const double lnT = log(T);")
+    w("const double invT = 1.0 / T;")
+    if rev:
+        L.emit_gibbs_prologue(w)
+    if L.uses_ctot():
+        # the total concentration needs every species up front; the values
+        # are loaded again where each species opens
+        w("double Ctot = 0.0;")
+        w("#pragma unroll")
+        w("for (int k = 0; k < NS; ++k) Ctot += kInvW[k] * rho * Y[k * ldY + i];")
+    # Yg: Y as a global-memory pointer that is passed through an empty asm
+    # before each window opens species, so the optimiser cannot reuse the
+    # values of the pass above or hoist the loads of later windows
+    w("pyro::gptr Yg = (pyro::gptr)Y;")
+    w("double C[NS], E[NS], w[NS];")
+    order = reaction_order(L, L.tuning.get("order_tries", 8))
+    uses = [species_uses(L, r) for r in m.reactions]
+    last_w = {}
+    for pos, ri in enumerate(order):
+        for k in uses[ri][2]:
+            last_w[k] = pos
+    open_c, open_e, open_w = set(), set(), set()
+    pending = set()
+    fence = L.tuning.get("lazy_fence", 4)
+    for pos, ri in enumerate(order):
+        r = m.reactions[ri]
+        c_u, e_u, w_u = uses[ri]
+        if c_u - open_c:
+            w('asm volatile("" : "+s"(Yg));')
+        for k in sorted(c_u - open_c):
+            w(f"C[{k}] = kInvW[{k}] * rho * Yg[{k} * ldY + i];")
+        for k in sorted(e_u - open_e):
+            L.emit_gibbs_one(w, k)
+        for k in sorted(w_u - open_w):
+            w(f"w[{k}] = 0.0;")
+        open_c |= c_u
+        open_e |= e_u
+        open_w |= w_u
+        w(f"{{  // R{ri}: {r.equation} [{r.reaction_type}]")
+        w.ind += 1
+        L.kf_block(w, r)
+        rf = L.prod_expr(L.reactant_pairs(r))
+        if r.reversible:
+            rr = L.prod_expr([(L.idx[sp], nu) for sp, nu in r.products])
+            w(f"const double q = kf * ({rf} - {L.kinv_expr(r)} * {rr});")
+        else:
+            w(f"const double q = kf * ({rf});")
+        upd = L.species_updates(r, "q")
+        for k, nu in upd:
+            if nu == 1.0:
+                w(f"w[{k}] += q;")
+            elif nu == -1.0:
+                w(f"w[{k}] -= q;")
+            else:
+                w(f"w[{k}] = fma({lit(nu)}, q, w[{k}]);")
+        pending.update(k for k, _ in upd)
+        w.ind -= 1
+        w("}")
+        done = sorted(k for k in w_u if last_w[k] == pos)
+        if (pos + 1) % L.tuning.get("pin_every", 1) == 0 or done \
+                or pos + 1 == len(order):
+            L.pin(w, [f"w[{k}]" for k in sorted(pending)])
+            pending.clear()
+        for k in done:
+            w(f"W[{k} * ldW + i] = w[{k}];")
+        if fence and (pos + 1) % fence == 0:
+            # keep the loads and stores of each window in place
+            w('asm volatile("" ::: "memory");')
+    for k in range(L.ns):
+        if k not in open_w:
+            w(f"W[{k} * ldW + i] = 0.0;")
+    w.ind -= 1
+    w("}")
+    w("")
+    if L.tab is not None:
+        tab = L.tab or [0.0]
+        decl = ("__constant__ double kTabWdotLazy[] = {"
                 + ", ".join(lit(x) for x in tab) + "};")
         w.lines.insert(table_at, decl)
         L.tab = None
@@ -807,6 +1035,133 @@ def _emit_wdot_jvp(w: _Writer, L: _Lowering):
             pending.clear()
         w.ind -= 1
         w("}")
+    w.ind -= 1
+    w("}")
+    w("")
+
+
+def _emit_wdot_jvp_lazy(w: _Writer, L: _Lowering):
+    """The forward-mode pass with species opened and closed along
+    ``reaction_order`` as in ``_emit_wdot_lazy``: C_k and dC_k loaded, E_k
+    and h_k/RT computed at first use, w_k and dw_k stored after their last
+    update."""
+    m = L.m
+    w("// omega-dot and its directional derivative, species opened and "
+      "closed along the reactions")
+    sig = ("__device__ __forceinline__ void net_production_rates_jvp_lazy("
+           "const double rho, const double T, const double* __restrict__ Y, "
+           "const int64_t ldY, const double a, const double dT, "
+           "const double* __restrict__ vY, const int64_t ldV, "
+           "const int64_t i, double* __restrict__ wout, "
+           "double* __restrict__ dwout, const int64_t ldO) {")
+    rev = any(r.reversible for r in m.reactions)
+    dc = ("kInvW[{k}] * fma(Y[{k} * ldY + i], a, "
+          "vY ? rho * vY[{k} * ldV + i] : 0.0)")
+    # branch-free form for the lazy pass: without vY the tangent term reads
+    # Y and multiplies it by zero
+    dcg = ("kInvW[{k}] * fma(Yg[{k} * ldY + i], a, "
+           "sv * Vg[{k} * ldVv + i])")
+    if not L.tuning.get("lazy", True) or (rev and not L.species_kc):
+        w(sig)
+        w.ind += 1
+        w("double C[NS], dC[NS], w[NS], dw[NS];")
+        w("#pragma unroll")
+        w("for (int k = 0; k < NS; ++k) {")
+        w("  C[k] = kInvW[k] * rho * Y[k * ldY + i];")
+        w("  dC[k] = " + dc.format(k="k") + ";")
+        w("}")
+        w("net_production_rates_jvp(T, C, dT, dC, w, dw);")
+        w("#pragma unroll")
+        w("for (int k = 0; k < NS; ++k) {")
+        w("  dwout[k * ldO + i] = dw[k];")
+        w("  if (wout) wout[k * ldO + i] = w[k];")
+        w("}")
+        w.ind -= 1
+        w("}")
+        w("")
+        return
+    w(sig)
+    w.ind += 1
+    w("const double lnT = log(T);")
+    w("const double invT = 1.0 / T;")
+    if rev:
+        L.emit_gibbs_prologue(w)
+    w("const double sv = vY ? rho : 0.0;")
+    w("const int64_t ldVv = vY ? ldV : ldY;")
+    w("pyro::gptr Yg = (pyro::gptr)Y;")
+    w("pyro::gptr Vg = (pyro::gptr)(vY ? vY : Y);")
+    if L.uses_ctot():
+        w("double Ctot = 0.0, dCtot = 0.0;")
+        w("#pragma unroll")
+        w("for (int k = 0; k < NS; ++k) {")
+        w("  Ctot += kInvW[k] * rho * Yg[k * ldY + i];")
+        w("  dCtot += " + dcg.format(k="k") + ";")
+        w("}")
+    w("double C[NS], dC[NS], E[NS], hrt[NS], w[NS], dw[NS];")
+    order = reaction_order(L, L.tuning.get("order_tries", 8))
+    uses = [species_uses(L, r) for r in m.reactions]
+    last_w = {}
+    for pos, ri in enumerate(order):
+        for k in uses[ri][2]:
+            last_w[k] = pos
+    open_c, open_e, open_w = set(), set(), set()
+    pending = set()
+    for pos, ri in enumerate(order):
+        r = m.reactions[ri]
+        c_u, e_u, w_u = uses[ri]
+        if c_u - open_c:
+            w('asm volatile("" : "+s"(Yg), "+s"(Vg));')
+        for k in sorted(c_u - open_c):
+            w(f"C[{k}] = kInvW[{k}] * rho * Yg[{k} * ldY + i];")
+            w(f"dC[{k}] = " + dcg.format(k=k) + ";")
+        for k in sorted(e_u - open_e):
+            L.emit_gibbs_one(w, k)
+            w(f"hrt[{k}] = {L.h_expr(m.species[k])};")
+        for k in sorted(w_u - open_w):
+            w(f"w[{k}] = 0.0; dw[{k}] = 0.0;")
+        open_c |= c_u
+        open_e |= e_u
+        open_w |= w_u
+        w(f"{{  // R{ri}: {r.equation} [{r.reaction_type}]")
+        w.ind += 1
+        L.kf_block_deriv(w, r)
+        dpf, dpb, cols = _reverse_parts(w, L, r)
+        fv = " + ".join(f"({e}) * dC[{j}]" for j, e in dpf.items()) or "0.0"
+        if r.reversible:
+            bv = " + ".join(f"({e}) * dC[{j}]" for j, e in dpb.items()) or "0.0"
+            w(f"const double ddelta = fma(-kinv, {bv}, {fv}) - dkinvT * dT * pb;")
+        else:
+            w(f"const double ddelta = {fv};")
+        dk = "dkfT * dT"
+        if r.kind in ("three-body", "falloff"):
+            w(f"const double dM = {L.third_body(r, 'dC', 'dCtot')};")
+            dk = "fma(dkfM, dM, dkfT * dT)"
+        w("const double q = kf * delta;")
+        w(f"const double dq = fma({dk}, delta, kf * ddelta);")
+        upd = L.species_updates(r, "q")
+        for k, nu in upd:
+            nl = lit(nu)
+            w(f"w[{k}] = fma({nl}, q, w[{k}]);")
+            w(f"dw[{k}] = fma({nl}, dq, dw[{k}]);")
+        pending.update(k for k, _ in upd)
+        w.ind -= 1
+        w("}")
+        done = sorted(k for k in w_u if last_w[k] == pos)
+        if (pos + 1) % L.tuning.get("pin_every", 1) == 0 or done \
+                or pos + 1 == len(order):
+            L.pin(w, [f"w[{k}]" for k in sorted(pending)]
+                  + [f"dw[{k}]" for k in sorted(pending)])
+            pending.clear()
+        for k in done:
+            w(f"dwout[{k} * ldO + i] = dw[{k}];")
+            w(f"if (wout) wout[{k} * ldO + i] = w[{k}];")
+        fence = L.tuning.get("lazy_fence", 4)
+        if fence and (pos + 1) % fence == 0:
+            w('asm volatile("" ::: "memory");')
+    for k in range(L.ns):
+        if k not in open_w:
+            w(f"dwout[{k} * ldO + i] = 0.0;")
+            w(f"if (wout) wout[{k} * ldO + i] = 0.0;")
     w.ind -= 1
     w("}")
     w("")
@@ -1146,9 +1501,11 @@ class HipCodeGenerator(CodeGenerator):
         _emit_prelude(w, L, name, mech.digest(), wdot_block)
         _emit_thermo(w, L)
         _emit_wdot(w, L)
+        _emit_wdot_lazy(w, L)
         _emit_reaction_quantities(w, L)
         _emit_creation_destruction(w, L)
         _emit_wdot_jvp(w, L)
+        _emit_wdot_jvp_lazy(w, L)
         _emit_wdot_vjp(w, L)
         _emit_reaction_cache(w, L)
         _emit_source_jacobian_rows(w, L)

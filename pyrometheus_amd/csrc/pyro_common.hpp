@@ -54,6 +54,9 @@ __device__ __forceinline__ double ipow(double x) {
 // Pointer into the constant address space: uniform reads through it become
 // scalar loads (one s_load_dwordx16 fetches eight doubles into SGPRs).
 typedef const __attribute__((address_space(4))) double* cptr;
+// Pointer into global memory (keeps loads through a pointer that passed an
+// asm statement on the global path rather than the flat one).
+typedef const __attribute__((address_space(1))) double* gptr;
 
 // Reciprocal from the hardware estimate plus two Newton steps (within an
 // ulp or so of 1/x for the positive normal operands it is used on).

@@ -190,18 +190,16 @@ k_reaction_quantities(const double* __restrict__ T,
   mech::reaction_quantities(T[i], c, i, ldR, kf, nlk, rf, rr, rnet);
 }
 
+// kWdotWaves > 0 asks the compiler for that many waves per SIMD (it then
+// budgets registers for that occupancy).
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK)
+__global__ void __launch_bounds__(BLOCK, mech::kWdotWaves > 0 ? mech::kWdotWaves : 1)
 k_wdot(const double* __restrict__ rho, const double* __restrict__ T,
        const double* __restrict__ Y, int64_t ldY, double* __restrict__ W,
        int64_t ldW, int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
   if (i >= n) return;
-  double C[mech::NS], w[mech::NS];
-  load_conc(rho, Y, ldY, i, C);
-  mech::net_production_rates(T[i], C, w);
-#pragma unroll
-  for (int k = 0; k < mech::NS; ++k) W[k * ldW + i] = w[k];
+  mech::net_production_rates_lazy(rho[i], T[i], Y, ldY, i, W, ldW);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -226,7 +224,7 @@ k_creation_destruction(const double* __restrict__ rho,
 // derivatives of w(rho, T, Y) with C_k = rho Y_k / W_k
 // ---------------------------------------------------------------------------
 // JVP: dC_k = (Y_k v_rho + rho v_Yk) / W_k, one forward-mode pass.
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, mech::kDerivWaves > 0 ? mech::kDerivWaves : 1)
 k_wdot_jvp(const double* __restrict__ rho, const double* __restrict__ T,
            const double* __restrict__ Y, int64_t ldY,
            const double* __restrict__ vr, const double* __restrict__ vT,
@@ -235,21 +233,9 @@ k_wdot_jvp(const double* __restrict__ rho, const double* __restrict__ T,
            int64_t n) {
   const int64_t i = cell_index();
   if (i >= n) return;
-  const double r = rho[i];
-  const double a = vr ? vr[i] : 0.0;
-  double C[mech::NS], dC[mech::NS], w[mech::NS], dw[mech::NS];
-#pragma unroll
-  for (int k = 0; k < mech::NS; ++k) {
-    const double y = Y[k * ldY + i];
-    C[k] = mech::kInvW[k] * r * y;
-    dC[k] = mech::kInvW[k] * fma(y, a, vY ? r * vY[k * ldV + i] : 0.0);
-  }
-  mech::net_production_rates_jvp(T[i], C, vT ? vT[i] : 0.0, dC, w, dw);
-#pragma unroll
-  for (int k = 0; k < mech::NS; ++k) {
-    dwout[k * ldO + i] = dw[k];
-    if (wout) wout[k * ldO + i] = w[k];
-  }
+  mech::net_production_rates_jvp_lazy(rho[i], T[i], Y, ldY, vr ? vr[i] : 0.0,
+                                      vT ? vT[i] : 0.0, vY, ldV, i, wout,
+                                      dwout, ldO);
 }
 
 // VJP: Cb = wb^T dw/dC, Tb = wb^T dw/dT in one reverse pass, then
