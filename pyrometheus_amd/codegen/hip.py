@@ -1224,6 +1224,159 @@ def _emit_wdot_vjp(w: _Writer, L: _Lowering):
     w("")
 
 
+def _emit_wdot_vjp_lazy(w: _Writer, L: _Lowering):
+    """The reverse-mode pass with species opened and closed along
+    ``reaction_order``: Y_k, C_k and the cotangent wb_k loaded and E_k,
+    h_k/RT computed at first use; the cotangent Cb_k of C_k is final (up to
+    the default-efficiency term) after the last reaction that reads C_k, and
+    its contributions g_Yk = Cb_k rho / W_k and sum_k Cb_k Y_k / W_k to g_rho
+    are taken there.  The default-efficiency cotangent mbd (added to every
+    Cb_k) is known only at the end: it is added to the stored g_Y in a last
+    read-modify-write pass and to g_rho through Ctot."""
+    m = L.m
+    w("// cotangent of omega-dot pulled back to (rho, T, Y), species opened "
+      "and closed along the reactions")
+    sig = ("__device__ __forceinline__ void net_production_rates_vjp_lazy("
+           "const double rho, const double T, const double* __restrict__ Y, "
+           "const int64_t ldY, const double* __restrict__ wbp, "
+           "const int64_t ldB, const int64_t i, double* __restrict__ g_rho, "
+           "double* __restrict__ g_T, double* __restrict__ g_Y, "
+           "const int64_t ldG) {")
+    rev = any(r.reversible for r in m.reactions)
+    if not L.tuning.get("lazy", True) or (rev and not L.species_kc):
+        w(sig)
+        w.ind += 1
+        w("double C[NS], b[NS], Cb[NS], Tb;")
+        w("#pragma unroll")
+        w("for (int k = 0; k < NS; ++k) {")
+        w("  C[k] = kInvW[k] * rho * Y[k * ldY + i];")
+        w("  b[k] = wbp[k * ldB + i];")
+        w("}")
+        w("net_production_rates_vjp(T, C, b, Cb, Tb);")
+        w("double gr = 0.0;")
+        w("#pragma unroll")
+        w("for (int k = 0; k < NS; ++k) {")
+        w("  gr = fma(Cb[k] * kInvW[k], Y[k * ldY + i], gr);")
+        w("  if (g_Y) g_Y[k * ldG + i] = Cb[k] * rho * kInvW[k];")
+        w("}")
+        w("if (g_rho) g_rho[i] = gr;")
+        w("if (g_T) g_T[i] = Tb;")
+        w.ind -= 1
+        w("}")
+        w("")
+        return
+    w(sig)
+    w.ind += 1
+    w("const double lnT = log(T);")
+    w("const double invT = 1.0 / T;")
+    if rev:
+        L.emit_gibbs_prologue(w)
+    w("pyro::gptr Yg = (pyro::gptr)Y;")
+    w("pyro::gptr Bg = (pyro::gptr)wbp;")
+    uses_mbd = any(r.kind in ("three-body", "falloff")
+                   and r.default_efficiency != 0.0 for r in m.reactions)
+    if L.uses_ctot():
+        w("double Ctot = 0.0;")
+        w("#pragma unroll")
+        w("for (int k = 0; k < NS; ++k) Ctot += kInvW[k] * rho * Yg[k * ldY + i];")
+    w("double y[NS], C[NS], E[NS], hrt[NS], wb[NS], Cb[NS];")
+    w("double tb = 0.0, mbd = 0.0, gr = 0.0;")
+    order = reaction_order(L, L.tuning.get("order_tries", 8))
+    uses = [species_uses(L, r) for r in m.reactions]
+    last_c = {}
+    for pos, ri in enumerate(order):
+        for k in uses[ri][0]:
+            last_c[k] = pos
+    open_c, open_e, open_b = set(), set(), set()
+    pending = set()
+    fence = L.tuning.get("lazy_fence", 4)
+    for pos, ri in enumerate(order):
+        r = m.reactions[ri]
+        c_u, e_u, w_u = uses[ri]
+        if (c_u - open_c) or (w_u - open_b):
+            w('asm volatile("" : "+s"(Yg), "+s"(Bg));')
+        for k in sorted(c_u - open_c):
+            w(f"y[{k}] = Yg[{k} * ldY + i];")
+            w(f"C[{k}] = kInvW[{k}] * rho * y[{k}];")
+            w(f"Cb[{k}] = 0.0;")
+        for k in sorted(w_u - open_b):
+            w(f"wb[{k}] = Bg[{k} * ldB + i];")
+        for k in sorted(e_u - open_e):
+            L.emit_gibbs_one(w, k)
+            w(f"hrt[{k}] = {L.h_expr(m.species[k])};")
+        open_c |= c_u
+        open_e |= e_u
+        open_b |= w_u
+        w(f"{{  // R{ri}: {r.equation} [{r.reaction_type}]")
+        w.ind += 1
+        upd = L.species_updates(r, "q")
+        rb = " + ".join(f"{lit(nu)} * wb[{k}]" for k, nu in upd) or "0.0"
+        w(f"const double rb = {rb};")
+        L.kf_block_deriv(w, r)
+        dpf, dpb, cols = _reverse_parts(w, L, r)
+        if r.reversible:
+            w("tb = fma(rb, fma(dkfT, delta, -kf * dkinvT * pb), tb);")
+        else:
+            w("tb = fma(rb, dkfT * delta, tb);")
+        w("const double g = rb * kf;")
+        for j in cols:
+            a, b = dpf.get(j), dpb.get(j)
+            if a is not None and b is not None:
+                w(f"Cb[{j}] = fma(g, fma(-kinv, {b}, {a}), Cb[{j}]);")
+            elif a is not None:
+                w(f"Cb[{j}] = fma(g, {a}, Cb[{j}]);")
+            else:
+                w(f"Cb[{j}] = fma(-g * kinv, {b}, Cb[{j}]);")
+        touched = set(cols)
+        if r.kind in ("three-body", "falloff"):
+            w("const double mb = rb * dkfM * delta;")
+            for j, e in L.efficiency_terms(r):
+                w(f"Cb[{j}] = fma({L.c(e)}, mb, Cb[{j}]);")
+                touched.add(j)
+            if r.default_efficiency != 0.0:
+                w(f"mbd = fma({L.c(r.default_efficiency)}, mb, mbd);")
+        pending.update(touched)
+        w.ind -= 1
+        w("}")
+        done = sorted(k for k in c_u if last_c[k] == pos)
+        if (pos + 1) % L.tuning.get("pin_every", 1) == 0 or done \
+                or pos + 1 == len(order):
+            L.pin(w, [f"Cb[{k}]" for k in sorted(pending)] + ["tb", "mbd"])
+            pending.clear()
+        for k in done:
+            w(f"gr = fma(Cb[{k}] * kInvW[{k}], y[{k}], gr);")
+            w(f"if (g_Y) g_Y[{k} * ldG + i] = Cb[{k}] * rho * kInvW[{k}];")
+        if fence and (pos + 1) % fence == 0:
+            w('asm volatile("" ::: "memory");')
+    closed = set(last_c)
+    if uses_mbd:
+        # every Cb_k gets mbd: g_rho through sum_k Y_k / W_k = Ctot / rho,
+        # g_Y by a last pass over the stored values (read back through a
+        # pointer the optimiser cannot relate to the stores above)
+        w("gr = fma(mbd, Ctot / rho, gr);")
+        w("if (g_Y) {")
+        w.ind += 1
+        w("double* __restrict__ gy = g_Y;")
+        w('asm volatile("" : "+s"(gy));')
+        for k in range(L.ns):
+            if k in closed:
+                w(f"gy[{k} * ldG + i] = fma(mbd * rho, kInvW[{k}], "
+                  f"gy[{k} * ldG + i]);")
+            else:
+                w(f"gy[{k} * ldG + i] = mbd * rho * kInvW[{k}];")
+        w.ind -= 1
+        w("}")
+    else:
+        for k in range(L.ns):
+            if k not in closed:
+                w(f"if (g_Y) g_Y[{k} * ldG + i] = 0.0;")
+    w("if (g_rho) g_rho[i] = gr;")
+    w("if (g_T) g_T[i] = tb;")
+    w.ind -= 1
+    w("}")
+    w("")
+
+
 def cache_slots(L: _Lowering):
     """Slot of each cached per-reaction quantity in the source-Jacobian
     workspace: k_f, k_r = k_f/K_c (reversible), q, dq/dT, dq/d[M]
@@ -1507,6 +1660,7 @@ class HipCodeGenerator(CodeGenerator):
         _emit_wdot_jvp(w, L)
         _emit_wdot_jvp_lazy(w, L)
         _emit_wdot_vjp(w, L)
+        _emit_wdot_vjp_lazy(w, L)
         _emit_reaction_cache(w, L)
         _emit_source_jacobian_rows(w, L)
         dense = mech.n_species <= DENSE_JAC_MAX_SPECIES

@@ -248,23 +248,8 @@ k_wdot_vjp(const double* __restrict__ rho, const double* __restrict__ T,
            double* __restrict__ g_Y, int64_t ldG, int64_t n) {
   const int64_t i = cell_index();
   if (i >= n) return;
-  const double r = rho[i];
-  double C[mech::NS], b[mech::NS], Cb[mech::NS], Tb;
-#pragma unroll
-  for (int k = 0; k < mech::NS; ++k) {
-    C[k] = mech::kInvW[k] * r * Y[k * ldY + i];
-    b[k] = wb[k * ldB + i];
-  }
-  mech::net_production_rates_vjp(T[i], C, b, Cb, Tb);
-  double gr = 0.0;
-#pragma unroll
-  for (int k = 0; k < mech::NS; ++k) {
-    const double y = Y[k * ldY + i];
-    gr = fma(Cb[k] * mech::kInvW[k], y, gr);
-    if (g_Y) g_Y[k * ldG + i] = Cb[k] * r * mech::kInvW[k];
-  }
-  if (g_rho) g_rho[i] = gr;
-  if (g_T) g_T[i] = Tb;
+  mech::net_production_rates_vjp_lazy(rho[i], T[i], Y, ldY, wb, ldB, i, g_rho,
+                                      g_T, g_Y, ldG);
 }
 
 // Jacobian of the flamelet source S = [vd/rho; W_k w_k/rho] with respect to
