@@ -12,6 +12,11 @@ the kernel's average launch time, measured with HIP events on the stream it
 runs on), and the CPU baseline: the numpy oracle (the reference's op
 structure) timed on a bounded sample on this host.
 
+The ``derivatives`` object times the hand-written VJP and JVP of the net
+production rates and the full (ns+1)^2 flamelet-source Jacobian on 2^22
+GRI-3.0 cells (seed 3, cotangent seed 33), with a central-difference check
+of the VJP at h = 1e-6 T.
+
 The ``flamelets`` object of the same line solves 256 steady flamelets of the
 9-species H2/air mechanism (grid sizes uniform in [64, 1024] clustered about
 Z_st, chi_st log-uniform in [1, 1000] 1/s, fuel H2:N2 at 300 K, air at 800 K,
@@ -63,6 +68,8 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=96)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--deriv-cells", type=int, default=1 << 22)
+    ap.add_argument("--no-derivatives", action="store_true")
     ap.add_argument("--no-wdot", action="store_true",
                     help="skip the omega-dot leg (profiling the flamelets)")
     ap.add_argument("--traffic-json",
@@ -117,6 +124,89 @@ def cpu_baseline(mech, seconds, seed):
                       f"ufunc per term), 1 process, CPU "
                       f"{platform.processor() or platform.machine()}, "
                       f"{os.cpu_count()} host threads visible; JAX absent"}
+
+
+def derivatives_leg(args, device):
+    """Config-3 derivatives on GRI-3.0: the VJP of the net production rates
+    with respect to (rho, T, Y) for a cotangent w ~ N(0, 1), the JVP, and the
+    full (ns+1)^2 Jacobian of the flamelet source S = [vd/rho; W_k w_k/rho]
+    with respect to (h, Y) (vd = 0, entry-major layout), each timed with HIP
+    events on the launch stream; plus a central-difference check of the
+    VJP's temperature cotangent (h = 1e-6 T) on a sample of cells."""
+    import torch
+    from pyrometheus_amd import bundled, get_thermochem_class
+    from pyrometheus_amd.adjoint import (wdot_jvp, wdot_vjp, source_jacobian,
+                                         source_jacobian_workspace)
+    mech = bundled("gri30")
+    gas = get_thermochem_class(mech)(device=device)
+    n = args.deriv_cells
+    ns, nv = mech.n_species, mech.n_species + 1
+    T, p, Y = device_inputs(mech, n, 3, device)
+    rho = gas.get_density(p, T, Y)
+    g = torch.Generator(device=device).manual_seed(33)
+    wbar = torch.randn((ns, n), generator=g, dtype=torch.float64, device=device)
+    vT = torch.randn(n, generator=g, dtype=torch.float64, device=device)
+    vY = 1e-3 * torch.randn((ns, n), generator=g, dtype=torch.float64,
+                            device=device)
+    out = torch.empty((ns, n), dtype=torch.float64, device=device)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        ms = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms.append(a.elapsed_time(b))
+        return sum(ms) / len(ms)
+
+    res = {"workload": f"gri30, {n} cells (seed 3), cotangent N(0,1) "
+                       "(seed 33), vd = 0"}
+    print("[bench] derivatives", file=sys.stderr, flush=True)
+    ms_w = timed(lambda: gas.get_net_production_rates(rho, T, Y, out=out), 5)
+    ms_v = timed(lambda: wdot_vjp(gas, rho, T, Y, wbar), 5)
+    ms_j = timed(lambda: wdot_jvp(gas, rho, T, Y, None, vT, vY), 5)
+    res.update({"wdot_ms": ms_w, "vjp_ms": ms_v, "jvp_ms": ms_j,
+                "vjp_over_wdot": ms_v / ms_w, "jvp_over_wdot": ms_j / ms_w,
+                "vjp_mcells_s": n / ms_v / 1e3})
+    # central difference of <wbar, w(T)> in T against the VJP's g_T
+    sel = torch.arange(0, n, max(1, n // 4096), device=device)[:4096]
+    rs, ts, ys, ws = rho[sel], T[sel], Y[:, sel].contiguous(), wbar[:, sel]
+    _, g_t, _ = wdot_vjp(gas, rs, ts, ys, ws.contiguous())
+    h = 1e-6 * ts
+    fp = (ws * gas.get_net_production_rates(rs, ts + h, ys)).sum(0)
+    fm = (ws * gas.get_net_production_rates(rs, ts - h, ys)).sum(0)
+    fd = (fp - fm) / (2 * h)
+    scale = (ws.abs() * gas.get_net_production_rates(rs, ts, ys).abs()
+             ).sum(0) / ts
+    res["fd_check"] = {"quantity": "g_T vs central difference, h = 1e-6 T, "
+                                   "4096 cells",
+                       "max_err_over_scale": float(
+                           ((g_t - fd).abs() / scale).max().item())}
+    del wbar, vT, vY, out
+    torch.cuda.empty_cache()
+    jac = torch.empty((nv, nv, n), dtype=torch.float64, device=device)
+    work = source_jacobian_workspace(gas)
+    ms_J = timed(lambda: source_jacobian(gas, T, p, Y, None, layout="entry",
+                                         out=jac, work=work), 3)
+    bytes_cell = 8 * (nv * nv + ns + 2)
+    res["source_jacobian"] = {
+        "ms": ms_J, "mcells_s": n / ms_J / 1e3, "layout": "entry (nv, nv, N)",
+        "output_gb": 8 * nv * nv * n / 1e9,
+        "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                     "bytes_per_cell": bytes_cell,
+                     "achieved": bytes_cell * n / (ms_J * 1e-3) / 1e9,
+                     "frac": bytes_cell * n / (ms_J * 1e-3) / 1e9
+                     / HBM_PEAK_GBS}}
+    assert torch.isfinite(jac[:, :, :4096]).all().item()
+    del jac, work, T, p, Y, rho
+    torch.cuda.empty_cache()
+    return res
 
 
 def flamelet_leg(args, device):
@@ -294,6 +384,8 @@ def main():
         return
     if args.no_wdot:
         line = {"metric": METRIC, "n_gpus": 1, "data": "synthetic"}
+        if not args.no_derivatives:
+            line["derivatives"] = derivatives_leg(args, device)
         if not args.no_flamelets:
             line["flamelets"] = flamelet_leg(args, device)
         if not args.no_serving:
@@ -372,6 +464,8 @@ def main():
     }
     del rho, T, Y, out
     torch.cuda.empty_cache()
+    if not args.no_derivatives:
+        line["derivatives"] = derivatives_leg(args, device)
     if not args.no_flamelets:
         line["flamelets"] = flamelet_leg(args, device)
     if not args.no_serving:
