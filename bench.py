@@ -279,6 +279,8 @@ def flamelet_leg(args, device):
                     f"({mech.n_species} species), N in [64, 1024] "
                     "Z-clustered, chi_st log-uniform [1, 1000] 1/s, "
                     "T_ox 800 K, T_fu 300 K, 1 atm, equilibrium start",
+        "batched_launch": "each iteration replayed from a captured HIP "
+                          "graph (re-captured when the batch is re-packed)",
         "batched_solves_per_s": batched_sps,
         "plain_solves_per_s": plain_sps,
         "speedup_vs_plain": batched_sps / plain_sps,

@@ -42,10 +42,12 @@ k_flamelet_assemble(const double* __restrict__ phi, int64_t ldp,
                     const int64_t* __restrict__ off,
                     const double* __restrict__ bc,  // (nsys, 2, NV)
                     // (nsys,): bit 0 Crank-Nicolson (else steady), bit 1
-                    // the dissipation rate chi0 (else chi)
+                    // the dissipation rate chi0 (else chi), bit 2 (with
+                    // bit 0) a time step starts here: phi_prev = phi and
+                    // r_prev = R(phi) are stored first
                     const int* __restrict__ modes,
-                    const double* __restrict__ phi_prev,
-                    const double* __restrict__ r_prev,
+                    double* __restrict__ phi_prev,
+                    double* __restrict__ r_prev,
                     const double* __restrict__ dt,  // (nsys,)
                     double* __restrict__ rhs, double* __restrict__ L,
                     double* __restrict__ U, double* __restrict__ r_out,
@@ -61,6 +63,7 @@ k_flamelet_assemble(const double* __restrict__ phi, int64_t ldp,
   const double ai = la[i], bi = lb[i], ci = lc[i];
   const double mask = (first || last) ? 0.0 : 1.0;
   const int mode = mode0 & 1;
+  const bool start = (mode0 & 5) == 5;
   const double hdt = (mode == 1) ? 0.5 * dt[s] : 0.0;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -71,9 +74,19 @@ k_flamelet_assemble(const double* __restrict__ phi, int64_t ldp,
     if (first) r -= bc[(s * 2 + 0) * NV + v];
     if (last) r -= bc[(s * 2 + 1) * NV + v];
     if (r_out) r_out[v * ldp + i] = r;
-    if (mode == 1)
-      r = phi[v * ldp + i] - phi_prev[v * ldp + i] -
-          hdt * (r + r_prev[v * ldp + i]);
+    if (mode == 1) {
+      double pp, rp;
+      if (start) {
+        pp = phi[v * ldp + i];
+        rp = r;
+        phi_prev[v * ldp + i] = pp;
+        r_prev[v * ldp + i] = rp;
+      } else {
+        pp = phi_prev[v * ldp + i];
+        rp = r_prev[v * ldp + i];
+      }
+      r = phi[v * ldp + i] - pp - hdt * (r + rp);
+    }
     rhs[i * NV + v] = -r;
   }
   double* blk = J + i * NV * NV;
@@ -154,7 +167,7 @@ PYRO_API int pyro_flamelet_assemble(
     const double* chi, const double* chi0, const double* la,
     const double* lb, const double* lc,
     const int* sys, const int64_t* off, const double* bc, const int* modes,
-    const double* phi_prev, const double* r_prev, const double* dt,
+    double* phi_prev, double* r_prev, const double* dt,
     double* rhs, double* L, double* U, double* r_out, int64_t n,
     hipStream_t stream) {
   if (n == 0) return pyro::kOk;

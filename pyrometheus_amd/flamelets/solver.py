@@ -353,7 +353,8 @@ def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
                 equilibrate_growth=2.5, newton_norm="reference",
                 kernel_events: Optional[dict] = None,
                 stats: Optional[dict] = None,
-                jacobian: str = "auto") -> List[FlameletResult]:
+                jacobian: str = "auto",
+                graphs: Optional[bool] = None) -> List[FlameletResult]:
     """Solve every problem with the control flow of ``FlameletSolver.solve``
     applied to each flamelet independently (preceded, when
     ``equilibrate_steps`` > 0, by ``FlameletSolver.equilibrate``); one launch
@@ -362,7 +363,9 @@ def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
     ``update_norm``; ``kernel_events``, if a dict, collects HIP events around
     every kernel (``FlameletBatch.kernel_ms``); ``stats``, if a dict,
     receives the iteration count and the sum of batch points over them;
-    ``jacobian`` picks the source-Jacobian kernels (``FlameletBatch``)."""
+    ``jacobian`` picks the source-Jacobian kernels (``FlameletBatch``);
+    ``graphs`` replays each iteration's launches from a captured HIP graph
+    (default: on a GPU unless ``kernel_events`` is given)."""
     from .stepper import FlameletStepper, SolveSettings
     settings = SolveSettings(
         newton_maxiter=newton_maxiter, newton_tol=newton_tol,
@@ -373,7 +376,7 @@ def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
         equilibrate_growth=equilibrate_growth, newton_norm=newton_norm,
         jacobian=jacobian)
     st = FlameletStepper(pyro_gas, settings, verbosity=verbosity,
-                         kernel_events=kernel_events)
+                         kernel_events=kernel_events, graphs=graphs)
     handles = st.add(problems)
     results = st.run()
     if stats is not None:

@@ -69,7 +69,8 @@ class _Slot:
 class FlameletStepper:
     def __init__(self, gas, settings: Optional[SolveSettings] = None,
                  verbosity=False, device=None,
-                 kernel_events: Optional[dict] = None):
+                 kernel_events: Optional[dict] = None,
+                 graphs: Optional[bool] = None):
         from .solver import NEWTON_NORMS
         self.gas = gas
         self.s = settings or SolveSettings()
@@ -87,6 +88,19 @@ class FlameletStepper:
         self.t0 = time.time()
         # when a dict is given, every batch records HIP events into it
         self.kernel_events = kernel_events
+        # replay the update's launch sequence from a captured HIP graph
+        # (captured again whenever the batch is re-packed); off while HIP
+        # events are recorded per kernel
+        if graphs is None:
+            graphs = self.dev.type == "cuda" and kernel_events is None
+        if graphs and kernel_events is not None:
+            raise ValueError("graph replay and per-kernel events exclude "
+                             "each other")
+        self.graphs = bool(graphs)
+        self._graph = None
+        self.graph_captures = 0
+        # when a list is given, step() appends the (B, 2) update norms
+        self.norm_log: Optional[list] = None
 
     # -- membership -------------------------------------------------------
     def add(self, problems: Sequence) -> List[int]:
@@ -185,7 +199,7 @@ class FlameletStepper:
         self.C = {k: np.array([sl.ctrl[k] for sl in self._slots],
                               dtype=float if k == "err_prev" else np.int64)
                   for k in _CTRL}
-        self.r_tmp = torch.empty_like(self.A["phi"])
+        self._graph = None
         # control block [modes | active | dt] of 4B int32 words, pinned on
         # the host; the previous upload from it has finished when step()
         # returns (step() reads the norms back before it returns)
@@ -224,26 +238,33 @@ class FlameletStepper:
         dts = np.where(eq, s.equilibrate_dt0 * s.equilibrate_growth
                        ** C["eq_step"], s.bdf_time_step)
         # the per-flamelet controls travel in one upload: modes (bit 0
-        # Crank-Nicolson, bit 1 the equilibration dissipation rate), the
-        # active flags and the time steps (b.dt is a view of the upload)
+        # Crank-Nicolson, bit 1 the equilibration dissipation rate, bit 2
+        # start of a time step), the active flags and the time steps (b.dt
+        # is a view of the upload)
+        # a flamelet starting a time step (bit 2) has the assembly kernel
+        # store phi_prev = phi and r_prev = R(phi) before it uses them
         nb, host = b.B, self._ctl_np
-        host[:nb] = cn.astype(np.int32) | (eq.astype(np.int32) << 1)
+        starting = cn & (C["inner"] == 0)
+        host[:nb] = (cn.astype(np.int32) | (eq.astype(np.int32) << 1)
+                     | (starting.astype(np.int32) << 2))
         host[nb:2 * nb] = live
         host[2 * nb:4 * nb].view(np.float64)[:] = dts
         self._ctl_dev.copy_(self._ctl_pin, non_blocking=True)
         modes, active = self._ctl_dev[:nb], self._ctl_dev[nb:2 * nb]
-        starting = cn & (C["inner"] == 0)
-        if starting.any():
-            b.residual(phi, chi, vd, tg, pres, self.r_tmp, modes,
-                       A["phi_prev"], A["r_prev"], chi0)
-            pm = self._pts(starting)
-            A["phi_prev"][:, pm] = phi[:, pm]
-            A["r_prev"][:, pm] = self.r_tmp[:, pm]
-        norm2, _ = b.newton_update(phi, chi, vd, tg, pres, modes,
-                                   A["phi_prev"], A["r_prev"], apply=True,
-                                   active=active, chi0=chi0,
-                                   refresh_guess=True)
-        delta = update_norm(norm2.cpu().numpy(), s.newton_norm, self.h_scale)
+        def update():
+            b.newton_update(phi, chi, vd, tg, pres, modes, A["phi_prev"],
+                            A["r_prev"], apply=True, active=active,
+                            chi0=chi0, refresh_guess=True)
+        if self.graphs:
+            if self._graph is None:
+                self._graph = self._capture(update)
+            self._graph.replay()
+        else:
+            update()
+        sums = b.norm2.cpu().numpy()
+        if self.norm_log is not None:
+            self.norm_log.append(sums.copy())
+        delta = update_norm(sums, s.newton_norm, self.h_scale)
         self.iterations += 1
         self.point_iterations += b.P
         now = time.time()
@@ -319,6 +340,25 @@ class FlameletStepper:
             if sizes[dead].sum() * 2 > sizes.sum():
                 self._pack()
         return out
+
+    def _capture(self, fn):
+        """Record the kernels ``fn`` launches into a HIP graph (nothing runs
+        during the capture).  Every array the kernels touch belongs to the
+        packed batch, so the graph is valid until the next re-pack; the
+        per-flamelet controls are read from device memory at replay."""
+        g = torch.cuda.CUDAGraph()
+        cur = torch.cuda.current_stream(self.dev)
+        side = torch.cuda.Stream(self.dev)  # default priority
+        side.wait_stream(cur)
+        with torch.cuda.stream(side):
+            g.capture_begin()
+            try:
+                fn()
+            finally:
+                g.capture_end()
+        cur.wait_stream(side)
+        self.graph_captures += 1
+        return g
 
     def run(self) -> Dict[int, object]:
         """Step until every admitted flamelet has finished."""

@@ -219,6 +219,37 @@ def test_gpu_batch_alone_equals_in_mix(gas, device):
 
 
 @pytest.mark.gpu
+def test_gpu_graph_replay_matches_eager(gas, device):
+    """Every iteration replayed from a captured HIP graph -- host work and a
+    synchronisation between replays, a new capture after each re-pack --
+    gives the update norms of every iteration and the final states of the
+    eager launches bit for bit."""
+    from pyrometheus_amd.flamelets.stepper import (FlameletStepper,
+                                                   SolveSettings)
+    probs = _problems(6, seed=4)
+    out = {}
+    for graphs in (False, True):
+        st = FlameletStepper(gas, SolveSettings(equilibrate_steps=24),
+                             graphs=graphs)
+        st.norm_log = []
+        handles = st.add(probs)
+        res = st.run()
+        out[graphs] = (st, [res[h] for h in handles])
+    (st_e, eager), (st_g, graph) = out[False], out[True]
+    assert st_e.graph_captures == 0 and st_g.graph_captures >= 1
+    assert len(st_g.norm_log) == len(st_e.norm_log) > 20
+    for it, (a, b) in enumerate(zip(st_e.norm_log, st_g.norm_log)):
+        assert np.array_equal(a, b), it
+    for j, (a, b) in enumerate(zip(eager, graph)):
+        assert a.converged == b.converged
+        for nm, u, v in (("h", a.state.enthalpy, b.state.enthalpy),
+                         ("Y", a.state.mass_fractions,
+                          b.state.mass_fractions),
+                         ("T", a.temperature, b.temperature)):
+            assert _same(u, v), (j, nm)
+
+
+@pytest.mark.gpu
 def test_gpu_batch_matches_oracle(gas, device):
     from pyrometheus_amd.flamelets.solver import solve_batch
     probs = _problems(3, seed=7, sizes=[21, 30, 17])
