@@ -224,6 +224,12 @@ def flamelet_leg(args, device):
                                        seed=args.flamelet_seed)
     kw = dict(equilibrate_steps=24, newton_norm="scaled")
     sizes = np.array([len(p.x) for p in probs])
+    # block-Thomas schedule for nv = 10 on this device, tuned on this
+    # workload's flamelet sizes (synthetic blocks)
+    from pyrometheus_amd.flamelets.autotune import tune_thomas
+    tuned = tune_thomas(gas, sizes=sizes, seed=args.flamelet_seed)
+    print(f"[bench] block-Thomas schedule: {tuned['kind']} "
+          f"alpha={tuned['alpha']:.3g}", file=sys.stderr, flush=True)
     print(f"[bench] flamelets: {len(probs)} batched, points {sizes.sum()}",
           file=sys.stderr, flush=True)
     solve_batch(gas, probs[:2], **kw)  # warm-up
@@ -281,6 +287,7 @@ def flamelet_leg(args, device):
                     "T_ox 800 K, T_fu 300 K, 1 atm, equilibrium start",
         "batched_launch": "each iteration replayed from a captured HIP "
                           "graph (re-captured when the batch is re-packed)",
+        "thomas_schedule": tuned,
         "batched_solves_per_s": batched_sps,
         "plain_solves_per_s": plain_sps,
         "speedup_vs_plain": batched_sps / plain_sps,

@@ -43,17 +43,18 @@ def _ptr(t):
     return None if t is None else t.data_ptr()
 
 
-def partition_table(sizes, part_len=None):
+def partition_table(sizes, part_len=None, alpha=None):
     """Cut systems of the given sizes into consecutive partitions of about
-    ``part_len`` rows (default sqrt(2 N) per system, at least 8): returns
-    numpy arrays pstart, pend, psys, jpart, jbefore, jafter, roff (see
-    ``pyro_flamelet_thomas_partitioned``)."""
+    ``part_len`` rows (default alpha sqrt(N) per system, at least 8, with
+    alpha = sqrt 2 unless given): returns numpy arrays pstart, pend, psys,
+    jpart, jbefore, jafter, roff (see ``pyro_flamelet_thomas_partitioned``)."""
     import math
+    a = math.sqrt(2.0) if alpha is None else float(alpha)
     pstart, pend, psys, jpart, jbefore, jafter = [], [], [], [], [], []
     roff = [0]
     o = 0
     for s, n in enumerate(sizes):
-        m = part_len or max(8, int(round(math.sqrt(2.0 * n))))
+        m = part_len or max(8, int(round(a * math.sqrt(n))))
         npart = max(1, -(-n // m))
         first = len(pstart)
         nj0 = roff[-1]
@@ -145,14 +146,35 @@ class FlameletBatch:
                                        device=self.dev)
         # kernel name -> list of (start, end) HIP events when timing is on
         self.events = None
-        # partitioned block Thomas (nv <= 32)
-        self.partitioned = (lib.has("pyro_flamelet_thomas_partitioned")
-                            and self.nv <= 32 and thomas != "sequential")
-        if self.partitioned:
-            self._setup_partitions(part_len)
+        # block-Thomas schedule: "auto" takes the one tuned for this block
+        # size and device (flamelets.autotune), "sequential" or
+        # "partitioned" force one
+        if thomas not in ("auto", "sequential", "partitioned"):
+            raise ValueError(f"unknown block-Thomas schedule {thomas!r}")
+        alpha = None
+        if thomas == "auto":
+            from .autotune import schedule
+            sch = schedule(self.nv, self.dev)
+            thomas, alpha = sch["kind"], sch["alpha"]
+        self.partitioned = False
+        self.set_thomas(thomas, alpha, part_len)
 
-    def _setup_partitions(self, part_len=None):
-        tabs = partition_table(self.sizes, part_len)
+    def supports_partitioned(self) -> bool:
+        return (self.lib.has("pyro_flamelet_thomas_partitioned")
+                and self.nv <= 32)
+
+    def set_thomas(self, kind: str, alpha: Optional[float] = None,
+                   part_len: Optional[int] = None):
+        """Use the sequential block Thomas or the partitioned one with
+        partitions of max(8, round(alpha sqrt(N))) rows (or ``part_len``);
+        the partitioned form falls back to the sequential one for nv > 32."""
+        self.partitioned = kind == "partitioned" and self.supports_partitioned()
+        self.part_alpha = alpha
+        if self.partitioned:
+            self._setup_partitions(part_len, alpha)
+
+    def _setup_partitions(self, part_len=None, alpha=None):
+        tabs = partition_table(self.sizes, part_len, alpha)
         t = lambda a: torch.as_tensor(a, device=self.dev)
         (self.p_start, self.p_end, self.p_sys, self.j_part, self.j_before,
          self.j_after, self.r_off) = (t(a) for a in tabs)

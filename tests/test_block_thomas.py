@@ -160,7 +160,8 @@ def test_gpu_partitioned_matches_oracle(part_len, device):
     grids = [np.linspace(0, 1, n) for n in sizes]
     y = np.zeros(nv - 1)
     y[-1] = 1.0
-    bp = FlameletBatch(gas, grids, y, y, part_len=part_len)
+    bp = FlameletBatch(gas, grids, y, y, thomas="partitioned",
+                       part_len=part_len)
     bs = FlameletBatch(gas, grids, y, y, thomas="sequential")
     assert bp.partitioned and not bs.partitioned
     P = sum(sizes)
@@ -188,3 +189,34 @@ def test_gpu_partitioned_matches_oracle(part_len, device):
         sc = np.max(np.abs(ref))
         assert np.max(np.abs(xp[o:o + n] - ref)) <= 1e-12 * sc, n
         assert np.max(np.abs(xp[o:o + n] - xs[o:o + n])) <= 1e-12 * sc, n
+
+
+@pytest.mark.gpu
+def test_gpu_autotuner_schedule(device):
+    """The block-Thomas autotuner times the sequential schedule and the
+    partitioned one per alpha on a synthetic batch, keeps the fastest for the
+    block size, and batches built afterwards use it."""
+    from pyrometheus_amd import bundled, get_thermochem_class
+    from pyrometheus_amd.flamelets import autotune
+    from pyrometheus_amd.flamelets.engine import FlameletBatch
+    gas = get_thermochem_class(bundled("sandiego"))(device=device)
+    y = np.zeros(gas.num_species)
+    y[-1] = 1.0
+    try:
+        res = autotune.tune_thomas(gas, sizes=[17, 40, 97, 300, 1000],
+                                   reps=2)
+        assert res["kind"] in ("sequential", "partitioned")
+        assert len(res["ms"]) == 1 + len(autotune.ALPHAS)
+        assert all(t > 0.0 for t in res["ms"].values())
+        best = min(res["ms"].values())
+        name = (res["kind"] if res["kind"] == "sequential"
+                else f"partitioned alpha={res['alpha']:.3g}")
+        assert res["ms"][name] == best
+        sch = autotune.schedule(10, device)
+        assert sch["kind"] == res["kind"]
+        b = FlameletBatch(gas, [np.linspace(0, 1, n) for n in (30, 200)],
+                          y, y)
+        assert b.partitioned == (res["kind"] == "partitioned")
+    finally:
+        autotune.reset()
+    assert autotune.schedule(10, device) == autotune.DEFAULT
