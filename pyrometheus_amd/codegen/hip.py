@@ -107,12 +107,15 @@ class _Writer:
 #:  wdot_waves / deriv_waves -- waves per SIMD the compiler must budget
 #:           registers for in k_wdot / k_wdot_jvp (0: its own choice);
 #:  lazy_fence -- an empty asm with a memory clobber every that many
-#:           reactions of the lazy passes.
+#:           reactions of the lazy passes;
+#:  prefetch -- the lazy omega-dot pass loads a species' mass fraction that
+#:           many reactions before the species opens;
+#:  wdot_block -- threads per block of k_wdot.
 #: (GRI-3.0, 2^24 cells: 16.6 ms with the species arrays held in registers,
 #: 12.2 ms lazy at two waves per SIMD.)
-DEFAULT_TUNING = {"pin": True, "pin_every": 2, "kc": "species", "ctable": True,
+DEFAULT_TUNING = {"pin": True, "pin_every": 3, "kc": "species", "ctable": True,
                   "lazy": True, "wdot_waves": 2, "deriv_waves": 0,
-                  "lazy_fence": 1, "order_tries": 8}
+                  "lazy_fence": 1, "order_tries": 8, "prefetch": 4}
 
 
 def element_shift(mech: Mechanism):
@@ -804,13 +807,25 @@ def _emit_wdot_lazy(w: _Writer, L: _Lowering):
     open_c, open_e, open_w = set(), set(), set()
     pending = set()
     fence = L.tuning.get("lazy_fence", 4)
+    # mass fractions are loaded ``prefetch`` reactions before the species
+    # opens, so the load's latency overlaps the reactions in between
+    dist = int(L.tuning.get("prefetch", 0))
+    opens_at = {}
+    seen = set()
+    for pos, ri in enumerate(order):
+        for k in sorted(uses[ri][0] - seen):
+            opens_at.setdefault(max(0, pos - dist), []).append(k)
+        seen |= uses[ri][0]
+    w("double yv[NS];")
     for pos, ri in enumerate(order):
         r = m.reactions[ri]
         c_u, e_u, w_u = uses[ri]
-        if c_u - open_c:
+        if pos in opens_at:
             w('asm volatile("" : "+s"(Yg));')
+            for k in opens_at[pos]:
+                w(f"yv[{k}] = Yg[{k} * ldY + i];")
         for k in sorted(c_u - open_c):
-            w(f"C[{k}] = kInvW[{k}] * rho * Yg[{k} * ldY + i];")
+            w(f"C[{k}] = kInvW[{k}] * rho * yv[{k}];")
         for k in sorted(e_u - open_e):
             L.emit_gibbs_one(w, k)
         for k in sorted(w_u - open_w):
@@ -1650,7 +1665,7 @@ class HipCodeGenerator(CodeGenerator):
         mech = as_mechanism(sol)
         L = _Lowering(mech, HipCodeGenerator.tuning)
         w = _Writer()
-        wdot_block = 256
+        wdot_block = int(L.tuning.get("wdot_block", 256))
         _emit_prelude(w, L, name, mech.digest(), wdot_block)
         _emit_thermo(w, L)
         _emit_wdot(w, L)

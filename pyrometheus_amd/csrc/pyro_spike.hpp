@@ -132,6 +132,156 @@ k_spike_sweep(const double* __restrict__ L, const double* __restrict__ C,
   if (singular && lane == 0 && n_singular) atomicAdd(n_singular, 1);
 }
 
+// Gauss-Jordan as pyro_bt::gauss_jordan_dpp for four independent systems
+// per wave, one per 16-lane row: lanes base..base+NV-1 (base = 16 g) hold
+// the rows of system g, gl = lane - base.  The pivot of each row of lanes
+// comes from its own DPP max-reduction; the pivot row's entries are read
+// with ds_bpermute from lane base + p, p differing between the systems.
+template <int NV, int W>
+__device__ __forceinline__ void gauss_jordan_grp(double (&a)[W], int lane,
+                                                 int& my_k, bool& singular) {
+  static_assert(NV <= 16, "one 16-lane row holds the rows");
+  const int base = lane & 48;
+  const int gl = lane - base;
+  const bool active = gl < NV;
+  {
+    double mx = 0.0;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) mx = fmax(mx, fabs(a[c]));
+    const double sc = mx > 0.0 ? pyro::recip(mx) : 1.0;
+#pragma unroll
+    for (int c = 0; c < W; ++c) a[c] *= sc;
+  }
+  unsigned used = 0u;
+  constexpr unsigned all = (1u << NV) - 1u;
+  my_k = -1;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double key = -1.0;
+    if (active && !((used >> gl) & 1u)) {
+      const double v = fabs(a[k]);
+      key = (v == v) ? v : -0.5;  // a NaN row is taken only as a last resort
+    }
+    long long kb = __double_as_longlong(key);
+    if (key >= 0.0) kb = (kb & ~0x3Fll) | (63 - lane);
+    // every lane of the 16-lane row holds the row's maximum
+    const double best = pyro_bt::row16_max(__longlong_as_double(kb));
+    int p;
+    if (best >= 0.0)
+      p = 63 - static_cast<int>(__double_as_longlong(best) & 0x3F) - base;
+    else
+      p = __builtin_ctz(~used & all);
+    used |= 1u << p;
+    const int src = base + p;
+    const double piv = __shfl(a[k], src);
+    singular = singular || !(fabs(piv) > 0.0);
+    const double inv = pyro::recip(piv);
+    const double g = (gl == p) ? 1.0 - inv : a[k] * inv;
+#pragma unroll
+    for (int j = k + 1; j < W; ++j) a[j] = fma(-g, __shfl(a[j], src), a[j]);
+    if (gl == p) my_k = k;
+  }
+}
+
+// k_spike_sweep for NV <= 16 with four sweeps per wave: sweep task
+// t = 4 blockIdx.x + lane / 16 is (partition t / 2, down when t is even),
+// the same work as blockIdx t of k_spike_sweep.  Consecutive tasks are the
+// two sweeps of neighbouring partitions of one system, so the four sweeps
+// of a wave have about the same length.
+template <int NV>
+__global__ void __launch_bounds__(64)
+k_spike_sweep_grp(const double* __restrict__ L, const double* __restrict__ C,
+                  const double* __restrict__ U, const double* __restrict__ R,
+                  const int64_t* __restrict__ off,
+                  const int64_t* __restrict__ pstart,
+                  const int64_t* __restrict__ pend,
+                  const int* __restrict__ psys, int64_t nparts,
+                  double* __restrict__ fact, double* __restrict__ maps,
+                  int* __restrict__ n_singular) {
+  constexpr int W = 3 * NV + 1;
+  constexpr int MW = 2 * NV + 1;
+  const int lane = threadIdx.x;
+  const int gl = lane & 15;
+  const int64_t task = static_cast<int64_t>(blockIdx.x) * 4 + (lane >> 4);
+  if (task >= 2 * nparts) return;  // whole 16-lane rows leave together
+  const int64_t part = task >> 1;
+  const bool down = (task & 1) == 0;
+  const bool active = gl < NV;
+  const int r = active ? gl : 0;
+  const int64_t s = pstart[part], e = pend[part];
+  const int sy = psys[part];
+  const int64_t s0 = off[sy], s1 = off[sy + 1];
+  const int64_t m = e - s;
+  if (!down && s == s0) return;  // never used
+  bool singular = false;
+  auto row = [&](int64_t t) { return down ? s + t : e - 1 - t; };
+  auto slot_coef = [&](int64_t i) -> double {
+    if (down) return i < s1 - 1 ? U[i] : 0.0;
+    return i > s0 ? L[i - 1] : 0.0;
+  };
+  auto back_coef = [&](int64_t i) -> double {
+    return down ? L[i] : U[i - 1];
+  };
+  const int base = lane & 48;
+  double a[W];
+  int my_k = gl;
+  {
+    const int64_t i = row(0);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a[c] = C[(i * NV + r) * NV + c];
+    const double sc = slot_coef(i);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a[NV + c] = (c == r) ? sc : 0.0;
+    a[2 * NV] = R[i * NV + r];
+    double ec = 0.0;
+    if (down && s > s0) ec = -L[s - 1];
+    if (!down && e < s1) ec = -U[e - 1];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a[2 * NV + 1 + c] = (c == r) ? ec : 0.0;
+  }
+  for (int64_t t = 0; t < m; ++t) {
+    const int64_t i = row(t);
+    double cpre[NV], rpre = 0.0, bpre = 0.0, spre = 0.0;
+    if (t + 1 < m) {
+      const int64_t i1 = row(t + 1);
+#pragma unroll
+      for (int c = 0; c < NV; ++c) cpre[c] = C[(i1 * NV + r) * NV + c];
+      rpre = R[i1 * NV + r];
+      bpre = back_coef(i);
+      spre = slot_coef(i1);
+    }
+    gauss_jordan_grp<NV, W>(a, lane, my_k, singular);
+    if (down && active) {
+      double* f = fact + (i * NV + my_k) * MW;
+#pragma unroll
+      for (int c = 0; c < MW; ++c) f[c] = a[NV + c];
+    }
+    if (t + 1 == m) break;
+    const int k = active ? my_k : 0;
+    const int src = base + k;
+    const double l = bpre;
+#pragma unroll
+    for (int c = 0; c < NV; ++c)
+      a[c] = fma(-l, a[NV + c], __shfl(cpre[c], src));
+    a[2 * NV] = fma(-l, a[2 * NV], __shfl(rpre, src));
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a[2 * NV + 1 + c] *= -l;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a[NV + c] = (c == k) ? spre : 0.0;
+  }
+  if (active) {
+    double* mp = maps + ((part * 2 + (down ? 0 : 1)) * NV + my_k) * MW;
+    mp[0] = a[2 * NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      const double ext = a[2 * NV + 1 + c], nxt = -a[NV + c];
+      mp[1 + c] = down ? ext : nxt;
+      mp[1 + NV + c] = down ? nxt : ext;
+    }
+  }
+  if (singular && gl == 0 && n_singular) atomicAdd(n_singular, 1);
+}
+
 // Interface system: row j = interface between partitions q and q+1 of a
 // system, unknown Z_j = (b_q, t_{q+1}):
 //   b_q     - D_q t_{q+1} - B_q b_{q-1}          = a_q       (down map of q)
@@ -251,8 +401,15 @@ int solve(const double* L, const double* C, const double* U, const double* R,
           const int64_t* roff, int64_t nsys, int64_t nj, double* fact,
           double* maps, double* Lr, double* Cr, double* Ur, double* Rr,
           double* Z, double* rwork, int* n_singular, hipStream_t stream) {
-  k_spike_sweep<NV><<<static_cast<unsigned>(2 * nparts), 64, 0, stream>>>(
-      L, C, U, R, off, pstart, pend, psys, fact, maps, n_singular);
+  if constexpr (NV <= 16) {
+    k_spike_sweep_grp<NV>
+        <<<static_cast<unsigned>((2 * nparts + 3) / 4), 64, 0, stream>>>(
+            L, C, U, R, off, pstart, pend, psys, nparts, fact, maps,
+            n_singular);
+  } else {
+    k_spike_sweep<NV><<<static_cast<unsigned>(2 * nparts), 64, 0, stream>>>(
+        L, C, U, R, off, pstart, pend, psys, fact, maps, n_singular);
+  }
   if (nj > 0) {
     k_spike_interfaces<NV><<<static_cast<unsigned>(nj), 64, 0, stream>>>(
         maps, jpart, nj, Lr, Cr, Ur, Rr);
