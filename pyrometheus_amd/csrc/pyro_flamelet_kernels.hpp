@@ -214,35 +214,32 @@ PYRO_API int pyro_flamelet_update(double* phi, int64_t ldp, const double* x,
   return pyro::launch_status();
 }
 
-// Partitioned block Thomas for the flamelet blocks (nv <= 32): the same
+// Partitioned block Thomas for the flamelet blocks (nv <= 64): the same
 // system as pyro_flamelet_thomas, with each system cut into partitions
-// [pstart[p], pend[p]) of system psys[p] (consecutive within a system) and
-// the interfaces between consecutive partitions solved as a block-
-// tridiagonal system with 2nv x 2nv blocks (roff: interface offsets per
-// system; jpart[j]: left partition of interface j; jbefore/jafter[p]:
-// interface on either side of partition p or -1).  Work arrays: fact
-// P*nv*(2nv+1), maps nparts*2*nv*(2nv+1), Lr/Cr/Ur nj*(2nv)^2, Rr and Z
-// nj*2nv, rwork nj*2nv*(2nv+1) doubles.
+// [pstart[p], pend[p]) of system psys[p] (consecutive within a system; the
+// partitions of system s are roff[s] + s .. roff[s+1] + s), the unknowns at
+// the interfaces between consecutive partitions found by an nv-sized block
+// recursion over the partitions (roff: interface offsets per system;
+// jbefore/jafter[p]: interface on either side of partition p or -1).  Work
+// arrays: fact P*nv*(2nv+1), maps nparts*2*nv*(2nv+1),
+// red nparts*nv*(2nv+2), Z nj*2nv doubles.
 PYRO_API int pyro_flamelet_thomas_partitioned(
     const double* L, const double* C, const double* U, const double* R,
     double* X, const int64_t* off, int64_t nsys, const int64_t* pstart,
-    const int64_t* pend, const int* psys, int64_t nparts, const int* jpart,
+    const int64_t* pend, const int* psys, int64_t nparts,
     const int64_t* jbefore, const int64_t* jafter, const int64_t* roff,
-    int64_t nj, double* fact, double* maps, double* Lr, double* Cr,
-    double* Ur, double* Rr, double* Z, double* rwork, int* n_singular,
-    hipStream_t stream) {
+    int64_t nj, double* fact, double* maps, double* red, double* Z,
+    int* n_singular, hipStream_t stream) {
   if (nsys == 0 || nparts == 0) return pyro::kOk;
-  if constexpr (pyro_fl::NV > 32) {
+  if constexpr (pyro_fl::NV > 64) {
     return pyro::kUnsupported;
   } else {
     if (nsys < 0 || nparts < 0 || nj < 0 || !L || !C || !U || !R || !X ||
-        !off || !pstart || !pend || !psys || !jbefore || !jafter || !fact ||
-        !maps || (nj > 0 && (!jpart || !roff || !Lr || !Cr || !Ur || !Rr ||
-                             !Z || !rwork)))
+        !off || !pstart || !pend || !psys || !jbefore || !jafter || !roff ||
+        !fact || !maps || (nj > 0 && (!red || !Z)))
       return pyro::kInvalidArgument;
     return pyro_spike::solve<pyro_fl::NV>(
-        L, C, U, R, X, off, pstart, pend, psys, nparts, jpart, jbefore,
-        jafter, roff, nsys, nj, fact, maps, Lr, Cr, Ur, Rr, Z, rwork,
-        n_singular, stream);
+        L, C, U, R, X, off, pstart, pend, psys, nparts, jbefore, jafter,
+        roff, nsys, nj, fact, maps, red, Z, n_singular, stream);
   }
 }

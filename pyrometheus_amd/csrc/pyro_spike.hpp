@@ -14,10 +14,12 @@
 //         stores [X_U | X_r | X_W] of every row for the recovery,
 //   up:   rows e-1..s with the coupling to x_e carried the same way; gives
 //         x_s = a' + B' x_{s-1} + D' x_e.
-// With b_q = x_{e-1} of partition q and t_q = x_s of partition q, the
-// interface unknowns Z_j = (b_q, t_{q+1}) form a block-tridiagonal system
-// with 2NV x 2NV blocks, solved by k_block_thomas<2NV, false>; then every
-// partition recovers its rows from the stored factors:
+// With b_q = x_{e-1} and t_q = x_s of partition q the two maps read
+//   b_q = a_q  + B_q  b_{q-1} + D_q  t_{q+1}   (down map of q)
+//   t_q = a'_q + B'_q b_{q-1} + D'_q t_{q+1}   (up map of q),
+// a recursion over the partitions of a system with NV x NV blocks
+// (k_spike_reduced) gives the interface unknowns Z_j = (b_q, t_{q+1}); then
+// every partition recovers its rows from the stored factors:
 //   x_i = X_r,i + X_W,i x_{s-1} - X_U,i x_{i+1}.
 // The chain each wavefront walks is the partition length plus the number
 // of partitions instead of the system length.
@@ -282,56 +284,136 @@ k_spike_sweep_grp(const double* __restrict__ L, const double* __restrict__ C,
   if (singular && gl == 0 && n_singular) atomicAdd(n_singular, 1);
 }
 
-// Interface system: row j = interface between partitions q and q+1 of a
-// system, unknown Z_j = (b_q, t_{q+1}):
-//   b_q     - D_q t_{q+1} - B_q b_{q-1}          = a_q       (down map of q)
-//   t_{q+1} - B'_{q+1} b_q - D'_{q+1} t_{q+2}     = a'_{q+1}  (up map of q+1)
-// Blocks in k_block_thomas<2NV, false> layout: Lr[j-1] couples row j to
-// Z_{j-1} = (b_{q-1}, t_q), Ur[j] couples row j to Z_{j+1} = (b_{q+1},
-// t_{q+2}).  jpart[j] = global index of partition q.
+// Interface unknowns of one system per wave, lane = row (NV <= 64).
+// Forward over the system's partitions q = 0..Q-1:
+//   b_0 = beta_0 + G_0 t_1 with beta_0 = a_0, G_0 = D_0;
+//   q > 0: substituting b_{q-1} = beta_{q-1} + G_{q-1} t_q into the up map,
+//     (I - B'_q G_{q-1}) t_q = a'_q + B'_q beta_{q-1} + D'_q t_{q+1},
+//   solved by Gauss-Jordan for t_q = tau_q + H_q t_{q+1}; then the down map
+//   gives b_q = beta_q + G_q t_{q+1} with P = B_q G_{q-1},
+//     beta_q = a_q + B_q beta_{q-1} + P tau_q,  G_q = P H_q + D_q.
+// Back: t_{Q-1} = tau_{Q-1}; for q = Q-2..0, b_q = beta_q + G_q t_{q+1},
+//   t_q = tau_q + H_q t_{q+1}.  Z_j of the interface between q and q+1
+//   receives (b_q, t_{q+1}).  red holds per partition and row
+//   [beta | G row | tau | H row] (2 NV + 2 doubles).
 template <int NV>
 __global__ void __launch_bounds__(64)
-k_spike_interfaces(const double* __restrict__ maps,
-                   const int* __restrict__ jpart, int64_t nj,
-                   double* __restrict__ Lr, double* __restrict__ Cr,
-                   double* __restrict__ Ur, double* __restrict__ Rr) {
-  constexpr int N2 = 2 * NV;
+k_spike_reduced(const double* __restrict__ maps,
+                const int64_t* __restrict__ roff, double* __restrict__ red,
+                double* __restrict__ Z, int* __restrict__ n_singular) {
   constexpr int MW = 2 * NV + 1;
-  const int64_t j = blockIdx.x;
-  if (j >= nj) return;
-  const int q = jpart[j];
-  const double* dn = maps + (static_cast<int64_t>(q) * 2 + 0) * NV * MW;
-  const double* up = maps + (static_cast<int64_t>(q + 1) * 2 + 1) * NV * MW;
-  for (int e = threadIdx.x; e < N2 * N2; e += 64) {
-    const int rr = e / N2, cc = e % N2;
-    double cv, lv = 0.0, uv = 0.0;
-    if (rr < NV) {
-      const double* m = dn + rr * MW;
-      if (cc < NV) {
-        cv = (rr == cc) ? 1.0 : 0.0;
-        lv = -m[1 + cc];               // -B_q on b_{q-1}
-      } else {
-        cv = -m[1 + NV + (cc - NV)];   // -D_q on t_{q+1}
-      }
-    } else {
-      const double* m = up + (rr - NV) * MW;
-      if (cc < NV) {
-        cv = -m[1 + cc];               // -B'_{q+1} on b_q
-      } else {
-        cv = (rr == cc) ? 1.0 : 0.0;
-        uv = -m[1 + NV + (cc - NV)];   // -D'_{q+1} on t_{q+2}
-      }
+  constexpr int N2 = 2 * NV;
+  constexpr int RW = 2 * NV + 2;
+  constexpr int WA = 2 * NV + 1;
+  const int sy = blockIdx.x;
+  const int lane = threadIdx.x;
+  const bool active = lane < NV;
+  const int r = active ? lane : 0;
+  const int64_t j0 = roff[sy], j1 = roff[sy + 1];
+  const int64_t nq = j1 - j0 + 1;   // partitions of this system
+  if (nq < 2) return;
+  const int64_t q0 = j0 + sy;       // its first partition
+  // map row r of partition q: [a | B (on b_{q-1}) | D (on t_{q+1})]
+  auto mrow = [&](int64_t q, int dir) {
+    return maps + ((q * 2 + dir) * NV + r) * MW;
+  };
+  auto rrow = [&](int64_t q, int row) { return red + (q * NV + row) * RW; };
+  bool singular = false;
+  double beta, G[NV];
+  {
+    const double* md = mrow(q0, 0);
+    beta = md[0];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) G[c] = md[1 + NV + c];
+    if (active) {
+      double* o = rrow(q0, r);
+      o[0] = beta;
+#pragma unroll
+      for (int c = 0; c < NV; ++c) o[1 + c] = G[c];
     }
-    Cr[j * N2 * N2 + e] = cv;
-    Ur[j * N2 * N2 + e] = uv;
-    // Lr[j-1] holds row j's coupling to Z_{j-1}; the first interface of a
-    // system has none (its B_q is zero)
-    if (j > 0) Lr[(j - 1) * N2 * N2 + e] = lv;
   }
-  if (threadIdx.x < N2) {
-    const int rr = threadIdx.x;
-    Rr[j * N2 + rr] = rr < NV ? dn[rr * MW] : up[(rr - NV) * MW];
+  for (int64_t q = q0 + 1; q < q0 + nq; ++q) {
+    const double* mu = mrow(q, 1);
+    const double* md = mrow(q, 0);
+    // [I - B' G | a' + B' beta | D'] in row r
+    double aug[WA];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) aug[c] = (c == r) ? 1.0 : 0.0;
+    double rhs = mu[0];
+#pragma unroll
+    for (int mm = 0; mm < NV; ++mm) {
+      const double bp = mu[1 + mm];
+      rhs = fma(bp, pyro_bt::bcast(beta, mm), rhs);
+#pragma unroll
+      for (int c = 0; c < NV; ++c)
+        aug[c] = fma(-bp, pyro_bt::bcast(G[c], mm), aug[c]);
+    }
+    aug[NV] = rhs;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) aug[NV + 1 + c] = mu[1 + NV + c];
+    // P = B G_{q-1} (row r) before G is replaced
+    double P[NV], bsum = md[0];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) P[c] = 0.0;
+#pragma unroll
+    for (int mm = 0; mm < NV; ++mm) {
+      const double bm = md[1 + mm];
+      bsum = fma(bm, pyro_bt::bcast(beta, mm), bsum);
+#pragma unroll
+      for (int c = 0; c < NV; ++c)
+        P[c] = fma(bm, pyro_bt::bcast(G[c], mm), P[c]);
+    }
+    int my_k, perm[NV];
+    pyro_bt::gauss_jordan_dpp<NV, WA>(aug, lane, my_k, perm, singular);
+    // lane perm[m] holds row m of [tau | H]
+    if (active) {
+      double* o = rrow(q, my_k);
+      o[1 + NV] = aug[NV];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) o[2 + NV + c] = aug[NV + 1 + c];
+    }
+    // G_{q-1} is no longer needed: G_q = D + P H accumulates in place
+#pragma unroll
+    for (int c = 0; c < NV; ++c) G[c] = md[1 + NV + c];
+#pragma unroll
+    for (int mm = 0; mm < NV; ++mm) {
+      const int src = perm[mm];
+      const double pm = P[mm];
+      bsum = fma(pm, pyro_bt::bcast(aug[NV], src), bsum);
+#pragma unroll
+      for (int c = 0; c < NV; ++c)
+        G[c] = fma(pm, pyro_bt::bcast(aug[NV + 1 + c], src), G[c]);
+    }
+    beta = bsum;
+    if (active) {
+      double* o = rrow(q, r);
+      o[0] = beta;
+#pragma unroll
+      for (int c = 0; c < NV; ++c) o[1 + c] = G[c];
+    }
   }
+  // back substitution; t in natural row order, lane r holds t[r]
+  const int64_t ql = q0 + nq - 1;
+  double t = rrow(ql, r)[1 + NV];
+  for (int64_t q = ql - 1; q >= q0; --q) {
+    const double* o = rrow(q, r);
+    double b = o[0];
+#pragma unroll
+    for (int mm = 0; mm < NV; ++mm) b = fma(o[1 + mm], pyro_bt::bcast(t, mm), b);
+    const int64_t j = j0 + (q - q0);
+    if (active) {
+      Z[j * N2 + r] = b;
+      Z[j * N2 + NV + r] = t;
+    }
+    if (q > q0) {
+      double tn = o[1 + NV];
+#pragma unroll
+      for (int mm = 0; mm < NV; ++mm)
+        tn = fma(o[2 + NV + mm], pyro_bt::bcast(t, mm), tn);
+      t = tn;
+    }
+  }
+  if (singular && lane == 0 && n_singular) atomicAdd(n_singular, 1);
 }
 
 // Recovery: x_i = X_r,i + X_W,i x_{s-1} - X_U,i x_{i+1}, from the last row
@@ -391,16 +473,15 @@ k_spike_recover(const double* __restrict__ fact,
   }
 }
 
-// Host side: the four launches.  All arrays are device arrays prepared by
+// Host side: the three launches.  All arrays are device arrays prepared by
 // the caller (see pyro_flamelet_thomas_partitioned).
 template <int NV>
 int solve(const double* L, const double* C, const double* U, const double* R,
           double* X, const int64_t* off, const int64_t* pstart,
           const int64_t* pend, const int* psys, int64_t nparts,
-          const int* jpart, const int64_t* jbefore, const int64_t* jafter,
-          const int64_t* roff, int64_t nsys, int64_t nj, double* fact,
-          double* maps, double* Lr, double* Cr, double* Ur, double* Rr,
-          double* Z, double* rwork, int* n_singular, hipStream_t stream) {
+          const int64_t* jbefore, const int64_t* jafter, const int64_t* roff,
+          int64_t nsys, int64_t nj, double* fact, double* maps, double* red,
+          double* Z, int* n_singular, hipStream_t stream) {
   if constexpr (NV <= 16) {
     k_spike_sweep_grp<NV>
         <<<static_cast<unsigned>((2 * nparts + 3) / 4), 64, 0, stream>>>(
@@ -410,14 +491,9 @@ int solve(const double* L, const double* C, const double* U, const double* R,
     k_spike_sweep<NV><<<static_cast<unsigned>(2 * nparts), 64, 0, stream>>>(
         L, C, U, R, off, pstart, pend, psys, fact, maps, n_singular);
   }
-  if (nj > 0) {
-    k_spike_interfaces<NV><<<static_cast<unsigned>(nj), 64, 0, stream>>>(
-        maps, jpart, nj, Lr, Cr, Ur, Rr);
-    pyro_bt::k_block_thomas<2 * NV, false>
-        <<<static_cast<unsigned>(nsys), 64, 0, stream>>>(Lr, Cr, Ur, Rr, Z,
-                                                         roff, rwork,
-                                                         n_singular);
-  }
+  if (nj > 0)
+    k_spike_reduced<NV><<<static_cast<unsigned>(nsys), 64, 0, stream>>>(
+        maps, roff, red, Z, n_singular);
   k_spike_recover<NV><<<static_cast<unsigned>(nparts), 64, 0, stream>>>(
       fact, pstart, pend, jbefore, jafter, Z, X);
   return pyro::launch_status();

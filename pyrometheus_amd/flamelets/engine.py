@@ -161,13 +161,13 @@ class FlameletBatch:
 
     def supports_partitioned(self) -> bool:
         return (self.lib.has("pyro_flamelet_thomas_partitioned")
-                and self.nv <= 32)
+                and self.nv <= 64)
 
     def set_thomas(self, kind: str, alpha: Optional[float] = None,
                    part_len: Optional[int] = None):
         """Use the sequential block Thomas or the partitioned one with
         partitions of max(8, round(alpha sqrt(N))) rows (or ``part_len``);
-        the partitioned form falls back to the sequential one for nv > 32."""
+        the partitioned form falls back to the sequential one for nv > 64."""
         self.partitioned = kind == "partitioned" and self.supports_partitioned()
         self.part_alpha = alpha
         if self.partitioned:
@@ -184,13 +184,8 @@ class FlameletBatch:
         dd = dict(dtype=torch.float64, device=self.dev)
         self.fact = torch.empty(self.P * nv * (2 * nv + 1), **dd)
         self.maps = torch.empty(self.n_parts * 2 * nv * (2 * nv + 1), **dd)
-        nj = max(self.n_j, 1)
-        self.Lr = torch.empty(nj * n2 * n2, **dd)
-        self.Cr = torch.empty(nj * n2 * n2, **dd)
-        self.Ur = torch.empty(nj * n2 * n2, **dd)
-        self.Rr = torch.empty(nj * n2, **dd)
-        self.Z = torch.empty(nj * n2, **dd)
-        self.rwork = torch.empty(nj * n2 * (n2 + 1), **dd)
+        self.red = torch.empty(self.n_parts * nv * (n2 + 2), **dd)
+        self.Z = torch.empty(max(self.n_j, 1) * n2, **dd)
 
     def time_kernels(self, on=True):
         """Record HIP events around every kernel launch (``kernel_ms``)."""
@@ -304,13 +299,10 @@ class FlameletBatch:
                        self.rhs.data_ptr(), self.x.data_ptr(),
                        self.off.data_ptr(), self.B, self.p_start.data_ptr(),
                        self.p_end.data_ptr(), self.p_sys.data_ptr(),
-                       self.n_parts, self.j_part.data_ptr(),
-                       self.j_before.data_ptr(), self.j_after.data_ptr(),
-                       self.r_off.data_ptr(), self.n_j,
-                       self.fact.data_ptr(), self.maps.data_ptr(),
-                       self.Lr.data_ptr(), self.Cr.data_ptr(),
-                       self.Ur.data_ptr(), self.Rr.data_ptr(),
-                       self.Z.data_ptr(), self.rwork.data_ptr(), None,
+                       self.n_parts, self.j_before.data_ptr(),
+                       self.j_after.data_ptr(), self.r_off.data_ptr(),
+                       self.n_j, self.fact.data_ptr(), self.maps.data_ptr(),
+                       self.red.data_ptr(), self.Z.data_ptr(), None,
                        _stream())
             return
         self._call("pyro_flamelet_thomas", self.L.data_ptr(),

@@ -186,9 +186,8 @@ MECH_SIGNATURES = {
                                _I64, _P),
     "pyro_flamelet_thomas": (_P, _P, _P, _P, _P, _P, _I64, _P, _P, _P),
     "pyro_flamelet_thomas_partitioned": (_P, _P, _P, _P, _P, _P, _I64, _P,
-                                         _P, _P, _I64, _P, _P, _P, _P, _I64,
-                                         _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                                         _P),
+                                         _P, _P, _I64, _P, _P, _P, _I64, _P,
+                                         _P, _P, _P, _P, _P),
     "pyro_flamelet_update": (_P, _I64, _P, _P, _P, _P, _P, _P, _I64, _P,
                              _I64, _P),
 }

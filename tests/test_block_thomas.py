@@ -145,17 +145,23 @@ def test_gpu_badly_scaled_rows(diag, device):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("part_len", [None, 2, 3, 8])
-def test_gpu_partitioned_matches_oracle(part_len, device):
+@pytest.mark.parametrize("mech,part_len", [("sandiego", None),
+                                           ("sandiego", 2), ("sandiego", 3),
+                                           ("sandiego", 8), ("gri30", None),
+                                           ("gri30", 5)])
+def test_gpu_partitioned_matches_oracle(mech, part_len, device):
     """The partitioned flamelet block Thomas (partitions swept from both
-    ends, interface system, recovery) against the oracle on ragged systems
-    with scalar couplings, and against the one-wavefront kernel."""
+    ends, the nv-sized recursion for the interface unknowns, recovery)
+    against the oracle on ragged systems with scalar couplings, and against
+    the one-wavefront kernel: nv = 10 (four sweeps per wavefront) and
+    nv = 54."""
     import torch
     from pyrometheus_amd import bundled, get_thermochem_class
     from pyrometheus_amd.flamelets.engine import FlameletBatch
-    gas = get_thermochem_class(bundled("sandiego"))(device=device)
-    nv = 10
-    sizes = [3, 4, 17, 64, 250, 1000, 5]
+    gas = get_thermochem_class(bundled(mech))(device=device)
+    nv = gas.num_species + 1
+    sizes = ([3, 4, 17, 64, 250, 1000, 5] if nv <= 16
+             else [3, 4, 17, 40, 90, 5])
     rng = np.random.default_rng(19)
     grids = [np.linspace(0, 1, n) for n in sizes]
     y = np.zeros(nv - 1)
