@@ -68,6 +68,7 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=96)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-flamelet-mechs", default="sandiego,gri30")
     ap.add_argument("--deriv-cells", type=int, default=1 << 22)
     ap.add_argument("--no-derivatives", action="store_true")
     ap.add_argument("--no-wdot", action="store_true",
@@ -320,10 +321,12 @@ def flamelet_leg(args, device):
 
 
 def serving_leg(args, device):
-    """Poisson arrivals of production-rate requests (2^10..2^20 cells,
-    9- and 53-species mechanisms) and flamelet solves (64..2048 points,
-    9-species), served by the continuous-batching server and by the plain
-    drop-in one request at a time (FIFO), on the same arrival trace."""
+    """Poisson arrivals of production-rate requests (2^10..2^20 cells) and
+    flamelet solves (64..2048 points), each on the 9- or the 53-species
+    mechanism with probability 1/2 (``--serve-flamelet-mechs`` restricts
+    the flamelets), served by the continuous-batching server and by the
+    plain drop-in one request at a time (FIFO), on the same arrival
+    trace."""
     import numpy as np
     import torch
     from pyrometheus_amd import bundled, get_thermochem_class
@@ -331,8 +334,9 @@ def serving_leg(args, device):
                                          serve_plain, summarize)
     gases = {n: get_thermochem_class(bundled(n))(device=device)
              for n in ("sandiego", "gri30")}
+    fmechs = [m for m in args.serve_flamelet_mechs.split(",") if m]
     arrivals = poisson_workload(gases, args.serve_requests, args.serve_rate,
-                                seed=5, flamelet_mechanisms=["sandiego"])
+                                seed=5, flamelet_mechanisms=fmechs)
     warm = poisson_workload(gases, 4, 100.0, seed=6, wdot_log2_cells=(10, 12),
                             flamelet_points=(64, 96),
                             flamelet_mechanisms=["sandiego"])
@@ -348,7 +352,8 @@ def serving_leg(args, device):
     out = {"workload": f"{len(arrivals)} Poisson arrivals at "
                        f"{args.serve_rate}/s (seed 5): 50% omega-dot of "
                        "2^U{10..20} cells on sandiego or gri30, 50% flamelet "
-                       "solves of U{64..2048} points on sandiego",
+                       "solves of U{64..2048} points on "
+                       + " or ".join(fmechs),
            "batched": summarize(lat, t_first, t_last),
            "batched_by_kind": {k: summarize(lat[kinds == k], t_first, t_last)
                                for k in ("wdot", "flamelet")},
