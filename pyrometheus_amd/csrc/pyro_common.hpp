@@ -58,6 +58,13 @@ typedef const __attribute__((address_space(4))) double* cptr;
 // asm statement on the global path rather than the flat one).
 typedef const __attribute__((address_space(1))) double* gptr;
 
+// Reciprocal from the hardware estimate (relative error below 5e-8 on
+// gfx950, measured) and one Newton step: relative error below 2.3e-15.
+__device__ __forceinline__ double recip1(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+
 // Reciprocal from the hardware estimate plus two Newton steps (within an
 // ulp or so of 1/x for the positive normal operands it is used on).
 __device__ __forceinline__ double recip(double x) {
