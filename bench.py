@@ -12,6 +12,9 @@ the kernel's average launch time, measured with HIP events on the stream it
 runs on), and the CPU baseline: the numpy oracle (the reference's op
 structure) timed on a bounded sample on this host.
 
+The ``pointwise`` object times the memory-bound methods (species h/RT,
+mixture c_p, T from e, k_f) on GRI-3.0 against HBM bandwidth.
+
 The ``derivatives`` object times the hand-written VJP and JVP of the net
 production rates and the full (ns+1)^2 flamelet-source Jacobian on 2^22
 GRI-3.0 cells (seed 3, cotangent seed 33), with a central-difference check
@@ -71,6 +74,7 @@ def parse():
     ap.add_argument("--serve-flamelet-mechs", default="sandiego,gri30")
     ap.add_argument("--deriv-cells", type=int, default=1 << 22)
     ap.add_argument("--no-derivatives", action="store_true")
+    ap.add_argument("--no-pointwise", action="store_true")
     ap.add_argument("--no-wdot", action="store_true",
                     help="skip the omega-dot leg (profiling the flamelets)")
     ap.add_argument("--traffic-json",
@@ -125,6 +129,72 @@ def cpu_baseline(mech, seconds, seed):
                       f"ufunc per term), 1 process, CPU "
                       f"{platform.processor() or platform.machine()}, "
                       f"{os.cpu_count()} host threads visible; JAX absent"}
+
+
+def pointwise_leg(args, device):
+    """The HBM-bound pointwise methods on GRI-3.0 over 2^24 cells (2^22 for
+    the per-reaction k_f): species h/RT, mixture c_p, T from the internal
+    energy (the reference's energy form with T_guess = T + U[-300, 300] K)
+    and k_f, each timed with HIP events on the launch stream, with the
+    compulsory bytes per cell and the resulting fraction of HBM bandwidth."""
+    import torch
+    from pyrometheus_amd import bundled, get_thermochem_class
+    mech = bundled("gri30")
+    gas = get_thermochem_class(mech)(device=device)
+    n = args.cells
+    ns, nr = mech.n_species, mech.n_reactions
+    T, p, Y = device_inputs(mech, n, args.seed, device)
+    g = torch.Generator(device=device).manual_seed(args.seed + 1)
+    tg = (T + 600.0 * (torch.rand(n, generator=g, dtype=torch.float64,
+                                  device=device) - 0.5)).clamp(200.0, 3500.0)
+    e = gas.get_mixture_internal_energy_mass(T, Y)
+    del p
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        ms = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms.append(a.elapsed_time(b))
+        return sum(ms) / len(ms)
+
+    def entry(ms, cells, bytes_cell):
+        gbs = bytes_cell * cells / (ms * 1e-3) / 1e9
+        return {"ms": ms, "cells": cells, "mcells_s": cells / ms / 1e3,
+                "bytes_per_cell": bytes_cell, "hbm_gbs": gbs,
+                "hbm_frac": gbs / HBM_PEAK_GBS}
+    print("[bench] pointwise", file=sys.stderr, flush=True)
+    res = {"workload": f"gri30, {n} cells (seed {args.seed})"}
+    res["species_enthalpies_rt"] = entry(
+        timed(lambda: gas.get_species_enthalpies_rt(T)), n, 8 * (ns + 1))
+    res["mixture_cp_mass"] = entry(
+        timed(lambda: gas.get_mixture_specific_heat_cp_mass(T, Y)), n,
+        8 * (ns + 2))
+    ms_t = timed(lambda: gas.get_temperature(e, tg, Y, True))
+    t_out = gas.get_temperature(e, tg, Y, True)
+    res["temperature_from_energy"] = dict(
+        entry(ms_t, n, 8 * (ns + 3)),
+        max_abs_err_K=float((t_out - T).abs().max().item()))
+    del e, tg, t_out
+    torch.cuda.empty_cache()
+    m = n // 4
+    Tm, Ym = T[:m].contiguous(), Y[:, :m].contiguous()
+    rho = gas.get_density(torch.full((m,), 101325.0, dtype=torch.float64,
+                                     device=device), Tm, Ym)
+    C = gas.get_concentrations(rho, Ym)
+    res["fwd_rate_coefficients"] = entry(
+        timed(lambda: gas.get_fwd_rate_coefficients(Tm, C)), m,
+        8 * (ns + 1 + nr))
+    del C, Tm, Ym, rho, T, Y
+    torch.cuda.empty_cache()
+    return res
 
 
 def derivatives_leg(args, device):
@@ -398,6 +468,8 @@ def main():
         return
     if args.no_wdot:
         line = {"metric": METRIC, "n_gpus": 1, "data": "synthetic"}
+        if not args.no_pointwise:
+            line["pointwise"] = pointwise_leg(args, device)
         if not args.no_derivatives:
             line["derivatives"] = derivatives_leg(args, device)
         if not args.no_flamelets:
@@ -478,6 +550,8 @@ def main():
     }
     del rho, T, Y, out
     torch.cuda.empty_cache()
+    if not args.no_pointwise:
+        line["pointwise"] = pointwise_leg(args, device)
     if not args.no_derivatives:
         line["derivatives"] = derivatives_leg(args, device)
     if not args.no_flamelets:

@@ -563,6 +563,7 @@ def _emit_prelude(w: _Writer, L: _Lowering, name: str, digest: str,
     w(f"constexpr int kWdotBlock = {wdot_block};")
     w(f"constexpr int kWdotWaves = {int(L.tuning.get('wdot_waves', 0))};")
     w(f"constexpr int kDerivWaves = {int(L.tuning.get('deriv_waves', 0))};")
+    w(f"constexpr int kTempWaves = {int(L.tuning.get('temp_waves', 0))};")
     w("__device__ constexpr double kW[NS] = {"
       + ", ".join(lit(x) for x in m.molecular_weights) + "};")
     w("__device__ constexpr double kInvW[NS] = {"
@@ -583,6 +584,54 @@ def _emit_thermo(w: _Writer, L: _Lowering):
                       L.dh_expr)
     L.emit_species_fn(w, "species_ds_r", "const double T, const double invT",
                       L.ds_expr)
+    # mixture sums sum_k yw_k (h_k/RT - shift) and sum_k yw_k (cp_k/R -
+    # shift), species by species: both NASA ranges evaluated, the result
+    # selected (no per-coefficient selects, no per-species arrays)
+    # The coefficients come from a constant table through a pointer passed
+    # through an empty asm before each species, so they are read where they
+    # are used (scalar loads) rather than hoisted out of the caller's
+    # Newton loop into registers.
+    table_at = len(w.lines)
+    w("__device__ __forceinline__ void mixture_h_cp_sums(const double T, "
+      "const double invT, const double (&yw)[NS], const double shift, "
+      "double& sh, double& sc) {")
+    w.ind += 1
+    L.tab = []
+    w("pyro::cptr K = (pyro::cptr)kTabMixHCp;")
+    L._flags(w)
+    w("sh = 0.0;")
+    w("sc = 0.0;")
+    for k, sp in enumerate(L.m.species):
+        def hpoly(c):
+            cs = [L.c(c[0])] + [L.c(c[j] / d) for j, d in ((1, 2), (2, 3),
+                                                            (3, 4), (4, 5))]
+            return f"fma({L.c(c[5])}, iTk, {L._horner(cs, 'Tk')})"
+
+        def cpoly(c):
+            return L._horner([L.c(c[j]) for j in range(5)], "Tk")
+        # copies of T and of the table pointer passed through an empty asm
+        # after the previous species' sums: the species are evaluated one
+        # after the other instead of all at once
+        w(f"{{  // {sp.name}")
+        w("  double Tk = T, iTk = invT;")
+        w('  asm volatile("" : "+v"(Tk), "+v"(iTk), "+s"(K));')
+        if sp.high == sp.low:
+            w(f"  sh = fma(yw[{k}], {hpoly(sp.high)} - shift, sh);")
+            w(f"  sc = fma(yw[{k}], {cpoly(sp.high)} - shift, sc);")
+        else:
+            f = L.tmid_flag[sp.t_mid]
+            w(f"  const double hh = {hpoly(sp.high)}, hl = {hpoly(sp.low)};")
+            w(f"  const double ch = {cpoly(sp.high)}, cl = {cpoly(sp.low)};")
+            w(f"  sh = fma(yw[{k}], ({f} ? hh : hl) - shift, sh);")
+            w(f"  sc = fma(yw[{k}], ({f} ? ch : cl) - shift, sc);")
+        w('  asm volatile("" : "+v"(sh), "+v"(sc));')
+        w("}")
+    w.ind -= 1
+    w("}")
+    tab = L.tab or [0.0]
+    w.lines.insert(table_at, "__constant__ double kTabMixHCp[] = {"
+                   + ", ".join(lit(x) for x in tab) + "};")
+    L.tab = None
     w("__device__ __forceinline__ void species_g_rt(const double T, "
       "const double lnT, const double invT, double (&o)[NS]) {")
     w.ind += 1

@@ -68,25 +68,6 @@ __device__ __forceinline__ double mass_average(const double* __restrict__ Y,
   return acc;
 }
 
-__device__ __forceinline__ void mix_h_cp(double t, const double* __restrict__ Y,
-                                         int64_t ldY, int64_t i, bool energy,
-                                         double& he, double& pv) {
-  // he = e or h (mass), pv = cv or cp (mass), from one pass over species
-  double h[mech::NS], cp[mech::NS];
-  mech::species_h_rt(t, 1.0 / t, h);
-  mech::species_cp_r(t, cp);
-  const double shift = energy ? 1.0 : 0.0;
-  double sh = 0.0, sc = 0.0;
-#pragma unroll
-  for (int k = 0; k < mech::NS; ++k) {
-    const double yw = Y[k * ldY + i] * mech::kInvW[k];
-    sh += yw * (h[k] - shift);
-    sc += yw * (cp[k] - shift);
-  }
-  he = kGasConstant * t * sh;
-  pv = kGasConstant * sc;
-}
-
 template <int WHICH>
 __global__ void __launch_bounds__(kBlock)
 k_mixture_thermo(const double* __restrict__ T, int64_t t_stride,
@@ -334,7 +315,7 @@ k_source_jacobian_rows(const double* __restrict__ T,
 // mode 0: step, then stop when |step| < tol (generated get_temperature)
 // mode 1: stop when |step| <= tol without taking it (JAX while_loop form)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, mech::kTempWaves > 0 ? mech::kTempWaves : 1)
 k_temperature(const double* __restrict__ target, int64_t target_stride,
               const double* __restrict__ t_guess, int64_t guess_stride,
               const double* __restrict__ Y, int64_t ldY,
@@ -345,12 +326,18 @@ k_temperature(const double* __restrict__ target, int64_t target_stride,
   if (i >= n) return;
   const double e = target[i * target_stride];
   double t = t_guess[i * guess_stride];
-  const bool energy = do_energy != 0;
+  const double shift = do_energy != 0 ? 1.0 : 0.0;
+  double yw[mech::NS];
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k) yw[k] = Y[k * ldY + i] * mech::kInvW[k];
   int it = 0;
   bool ok = false;
   for (; it < maxit; ++it) {
-    double he, pv;
-    mix_h_cp(t, Y, ldY, i, energy, he, pv);
+    // he = e or h (mass), pv = cv or cp (mass), one pass over the species
+    double sh, sc;
+    mech::mixture_h_cp_sums(t, 1.0 / t, yw, shift, sh, sc);
+    const double he = kGasConstant * t * sh;
+    const double pv = kGasConstant * sc;
     const double dt = (e - he) / pv;  // = -rhs/deriv with deriv = -pv
     if (mode == 1) {
       if (fabs(dt) <= tol) { ok = true; break; }
