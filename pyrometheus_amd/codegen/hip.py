@@ -568,6 +568,10 @@ def _emit_prelude(w: _Writer, L: _Lowering, name: str, digest: str,
       + ", ".join(lit(x) for x in m.molecular_weights) + "};")
     w("__device__ constexpr double kInvW[NS] = {"
       + ", ".join(lit(1.0 / x) for x in m.molecular_weights) + "};")
+    # the same values where a kernel must read them from memory (scalar
+    # loads) rather than have them folded into immediates
+    w("__constant__ double kInvWMem[NS] = {"
+      + ", ".join(lit(1.0 / x) for x in m.molecular_weights) + "};")
     w("using pyro::kGasConstant;")
     w("using pyro::kOneAtm;")
     w("")
@@ -1521,10 +1525,11 @@ def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
       "double (&C)[NS], const double rho, const double T, "
       "const double cp, const double mmw, const double vd, "
       "const double* __restrict__ work, const unsigned il, "
-      "const double* __restrict__ dTl, const int hs, "
+      "const double* dTl, const int hs, "
       "double* __restrict__ o, const int64_t s_rc, "
       "double* __restrict__ S, const int64_t ldS) {")
     w.ind += 1
+    w("pyro::cptr IW = (pyro::cptr)kInvWMem;")
     w("double irho = 1.0 / rho;")
     w("double rT = rho * (1.0 / T);")
     w("double dT0 = 1.0 / cp;")
@@ -1559,6 +1564,9 @@ def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
             w(f'asm volatile("" : {ops});')
         w('asm volatile("" : "+v"(rm), "+v"(rT), "+v"(dr0), "+v"(dT0), '
           '"+v"(irho));')
+        # fresh copies of the cache and LDS pointers: this row's loads cannot
+        # be hoisted above the previous row (and kept live meanwhile)
+        w('asm volatile("" : "+s"(work), "+v"(dTl), "+s"(IW));')
         w("double G[NS];")
         w("#pragma unroll")
         w("for (int j = 0; j < NS; ++j) G[j] = 0.0;")
@@ -1607,9 +1615,9 @@ def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
         w("#pragma unroll")
         w("for (int q = 0; q < NS; ++q) {")
         w("  const double dT = dTl[q * hs];")
-        w("  const double dr = fma(-rT, dT, rm * kInvW[q]);")
+        w("  const double dr = fma(-rT, dT, rm * IW[q]);")
         w(f"  row[(1 + q) * s_rc] = fma(al, dr, fma(be, dT, "
-          f"kW[{k}] * kInvW[q] * G[q]));")
+          f"kW[{k}] * IW[q] * G[q]));")
         w("}")
         w(f"if (S) S[{k + 1} * ldS] = f * wk;")
         # keep the next row's loads below this row's stores: one row's
