@@ -785,7 +785,7 @@ def _greedy_order(uses, rank) -> List[int]:
     return order
 
 
-def reaction_order(L: _Lowering, tries: int = 8) -> List[int]:
+def reaction_order(L: _Lowering, tries: int = 8, uses=None) -> List[int]:
     """An order of the reactions that keeps few species live at once: a
     species' concentration and Gibbs factor are opened by the first reaction
     that reads them and its rate is closed (stored) by the last that writes
@@ -795,7 +795,8 @@ def reaction_order(L: _Lowering, tries: int = 8) -> List[int]:
     peak is kept.  The net production rates are sums over reactions, so the
     order changes only their rounding."""
     import random
-    uses = [species_uses(L, r) for r in L.m.reactions]
+    if uses is None:
+        uses = [species_uses(L, r) for r in L.m.reactions]
     n = len(uses)
     best, best_peak = None, None
     for t in range(tries):
@@ -1704,6 +1705,116 @@ def _emit_creation_destruction(w: _Writer, L: _Lowering):
     w("")
 
 
+def _emit_creation_destruction_lazy(w: _Writer, L: _Lowering):
+    """Creation and destruction rates of one cell with species opened and
+    closed along a reaction order as in ``_emit_wdot_lazy``: C_k loaded and
+    g_k/RT computed at first use, both rates of a species stored after the
+    last reaction it takes part in (chem_expr.py:542-567)."""
+    m = L.m
+    w("// creation/destruction split, species opened and closed along the "
+      "reactions (chem_expr.py:542-567)")
+    w("__device__ __forceinline__ void creation_destruction_lazy("
+      "const double rho, const double T, const double* __restrict__ Y, "
+      "const int64_t ldY, const int64_t i, double* __restrict__ cdot, "
+      "double* __restrict__ ddot, const int64_t ldW) {")
+    w.ind += 1
+    if not L.tuning.get("lazy", True):
+        w("double C[NS], cd[NS], dd[NS];")
+        w("#pragma unroll")
+        w("for (int k = 0; k < NS; ++k) C[k] = kInvW[k] * rho * Y[k * ldY + i];")
+        w("creation_destruction(T, C, cd, dd);")
+        w("#pragma unroll")
+        w("for (int k = 0; k < NS; ++k) {")
+        w("  cdot[k * ldW + i] = cd[k];")
+        w("  ddot[k * ldW + i] = dd[k];")
+        w("}")
+        w.ind -= 1
+        w("}")
+        w("")
+        return
+    rev = any(r.reversible for r in m.reactions)
+    w("const double lnT = log(T);")
+    w("const double invT = 1.0 / T;")
+    if rev:
+        w("const double c0 = log(kOneAtm / (kGasConstant * T));")
+        L._flags(w)
+    if L.uses_ctot():
+        w("double Ctot = 0.0;")
+        w("#pragma unroll")
+        w("for (int k = 0; k < NS; ++k) Ctot += kInvW[k] * rho * Y[k * ldY + i];")
+    w("pyro::gptr Yg = (pyro::gptr)Y;")
+    w("double C[NS], g[NS], cd[NS], dd[NS];")
+    uses = []
+    for r in m.reactions:
+        c = {L.idx[sp] for sp, _ in r.reactants}
+        e = set()
+        spc = {L.idx[sp] for sp, _ in list(r.reactants) + list(r.products)}
+        if r.reversible:
+            c |= {L.idx[sp] for sp, _ in r.products}
+            e = set(spc)
+        if r.kind in ("three-body", "falloff"):
+            c |= {k for k, _ in L.efficiency_terms(r)}
+        uses.append((c, e, spc))
+    order = reaction_order(L, L.tuning.get("order_tries", 8), uses)
+    last_w = {}
+    for pos, ri in enumerate(order):
+        for k in uses[ri][2]:
+            last_w[k] = pos
+    open_c, open_e, open_w = set(), set(), set()
+    pending = set()
+    for pos, ri in enumerate(order):
+        r = m.reactions[ri]
+        c_u, e_u, w_u = uses[ri]
+        if c_u - open_c:
+            w('asm volatile("" : "+s"(Yg));')
+        for k in sorted(c_u - open_c):
+            w(f"C[{k}] = kInvW[{k}] * rho * Yg[{k} * ldY + i];")
+        for k in sorted(e_u - open_e):
+            sp = m.species[k]
+            w(f"g[{k}] = {L.h_expr(sp)} - ({L.s_expr(sp)});")
+        for k in sorted(w_u - open_w):
+            w(f"cd[{k}] = 0.0; dd[{k}] = 0.0;")
+        open_c |= c_u
+        open_e |= e_u
+        open_w |= w_u
+        w(f"{{  // R{ri}: {r.equation}")
+        w.ind += 1
+        L.kf_block(w, r)
+        w(f"const double qf = kf * ({L.prod_expr(L.reactant_pairs(r))});")
+        if r.reversible:
+            rr = L.prod_expr([(L.idx[sp], nu) for sp, nu in r.products])
+            w(f"const double qr = kf * exp({L.nlk_expr(r)}) * ({rr});")
+        for sp, nu in r.reactants:
+            k = L.idx[sp]
+            w(f"dd[{k}] = fma({lit(nu)}, qf, dd[{k}]);")
+            if r.reversible:
+                w(f"cd[{k}] = fma({lit(nu)}, qr, cd[{k}]);")
+        for sp, nu in r.products:
+            k = L.idx[sp]
+            w(f"cd[{k}] = fma({lit(nu)}, qf, cd[{k}]);")
+            if r.reversible:
+                w(f"dd[{k}] = fma({lit(nu)}, qr, dd[{k}]);")
+        pending |= w_u
+        w.ind -= 1
+        w("}")
+        done = sorted(k for k in w_u if last_w[k] == pos)
+        if (pos + 1) % L.tuning.get("pin_every", 1) == 0 or done \
+                or pos + 1 == len(order):
+            L.pin(w, [f"cd[{k}]" for k in sorted(pending)]
+                  + [f"dd[{k}]" for k in sorted(pending)])
+            pending.clear()
+        for k in done:
+            w(f"cdot[{k} * ldW + i] = cd[{k}];")
+            w(f"ddot[{k} * ldW + i] = dd[{k}];")
+    for k in range(L.ns):
+        if k not in open_w:
+            w(f"cdot[{k} * ldW + i] = 0.0;")
+            w(f"ddot[{k} * ldW + i] = 0.0;")
+    w.ind -= 1
+    w("}")
+    w("")
+
+
 class HipCodeGenerator(CodeGenerator):
     """``hip`` target: mechanism IR -> HIP C++ translation unit."""
 
@@ -1739,6 +1850,7 @@ class HipCodeGenerator(CodeGenerator):
         _emit_wdot_lazy(w, L)
         _emit_reaction_quantities(w, L)
         _emit_creation_destruction(w, L)
+        _emit_creation_destruction_lazy(w, L)
         _emit_wdot_jvp(w, L)
         _emit_wdot_jvp_lazy(w, L)
         _emit_wdot_vjp(w, L)

@@ -191,14 +191,7 @@ k_creation_destruction(const double* __restrict__ rho,
                        int64_t ldW, int64_t n) {
   const int64_t i = cell_index();
   if (i >= n) return;
-  double C[mech::NS], cd[mech::NS], dd[mech::NS];
-  load_conc(rho, Y, ldY, i, C);
-  mech::creation_destruction(T[i], C, cd, dd);
-#pragma unroll
-  for (int k = 0; k < mech::NS; ++k) {
-    cdot[k * ldW + i] = cd[k];
-    ddot[k * ldW + i] = dd[k];
-  }
+  mech::creation_destruction_lazy(rho[i], T[i], Y, ldY, i, cdot, ddot, ldW);
 }
 
 // ---------------------------------------------------------------------------
