@@ -104,27 +104,45 @@ class FlameletStepper:
 
     # -- membership -------------------------------------------------------
     def add(self, problems: Sequence) -> List[int]:
-        """Admit flamelets (``FlameletProblem``); returns their handles."""
+        """Admit flamelets (``FlameletProblem``); returns their handles.  The
+        problems' arrays go to the device in one copy per kind; each
+        flamelet's arrays are views of those."""
         s, dev, dd = self.s, self.dev, dict(dtype=torch.float64,
                                             device=self.dev)
+        problems = list(problems)
+        if not problems:
+            return []
         first = NEWTON_TRY if s.try_newton else MARCH
+        sizes = [len(p.x) for p in problems]
+        offs = np.concatenate([[0], np.cumsum(sizes)])
+        host = np.concatenate([np.asarray(p.phi, dtype=float)
+                               for p in problems], axis=1)
+        phi_all = torch.as_tensor(host, **dd)
+        guess_all = phi_all.clone()
+        spare = torch.empty((3,) + tuple(phi_all.shape), **dd)
+        pts = np.stack([
+            np.concatenate([np.asarray(p.t_guess, dtype=float)
+                            for p in problems]),
+            np.concatenate([np.asarray(p.chi, dtype=float) for p in problems]),
+            np.concatenate([np.zeros(n) if p.vd is None
+                            else np.asarray(p.vd, dtype=float)
+                            for p, n in zip(problems, sizes)]),
+            np.concatenate([np.full(n, float(p.pressure))
+                            for p, n in zip(problems, sizes)])])
+        chi0 = pts[1].copy()
+        for a, e in zip(offs[:-1], offs[1:]):
+            chi0[a + 1:e - 1] = 0.0  # equilibration: chi only at the ends
+        pt_all = torch.as_tensor(np.vstack([pts, chi0[None]]), **dd)
         out = []
-        for p in problems:
-            n = len(p.x)
-            phi = torch.as_tensor(np.asarray(p.phi), **dd).contiguous()
-            chi = torch.as_tensor(np.asarray(p.chi), **dd)
-            chi0 = chi.clone()
-            chi0[1:-1] = 0.0
+        for j, p in enumerate(problems):
+            a, e = int(offs[j]), int(offs[j + 1])
             arrays = {
-                "phi": phi, "phi_guess": phi.clone(),
-                "phi_prev": torch.empty_like(phi),
-                "r_prev": torch.empty_like(phi),
-                "phi_save": torch.empty_like(phi),
-                "tg": torch.as_tensor(np.asarray(p.t_guess), **dd).clone(),
-                "chi": chi, "chi0": chi0,
-                "vd": (torch.zeros(n, **dd) if p.vd is None
-                       else torch.as_tensor(np.asarray(p.vd), **dd)),
-                "pres": torch.full((n,), float(p.pressure), **dd),
+                "phi": phi_all[:, a:e], "phi_guess": guess_all[:, a:e],
+                "phi_prev": spare[0, :, a:e], "r_prev": spare[1, :, a:e],
+                "phi_save": spare[2, :, a:e],
+                "tg": pt_all[0, a:e], "chi": pt_all[1, a:e],
+                "vd": pt_all[2, a:e], "pres": pt_all[3, a:e],
+                "chi0": pt_all[4, a:e],
             }
             ctrl = dict(phase=EQUIL if s.equilibrate_steps > 0 else first,
                         it=0, err_prev=np.inf, step=0, inner=0, attempt=0,
@@ -157,24 +175,30 @@ class FlameletStepper:
         return 8 * (nv * nv + nv * (nv + 1) + (len(_STATE) + 4) * nv
                     + len(_POINT) + 10)
 
-    def _unpack(self):
-        if self.batch is None:
-            return
-        for j, sl in enumerate(self._slots):
-            a, e = int(self.off[j]), int(self.off[j + 1])
-            for k in _STATE:
-                sl.arrays[k] = self.A[k][:, a:e].clone()
-            for k in _POINT:
-                sl.arrays[k] = self.A[k][a:e].clone()
-            for k in _CTRL:
-                sl.ctrl[k] = self.C[k][j]
-        self.batch = None
-
     def _pack(self):
-        self._unpack()
-        self._slots = [sl for sl in self._slots
-                       if sl.ctrl["phase"] not in (DONE_OK, DONE_FAIL)]
-        self._slots += self._pending
+        """Build a new batch from the live flamelets of the current one (their
+        state gathered from the packed arrays in one index_select per array)
+        followed by the newly admitted ones."""
+        kept, gather = [], None
+        if self.batch is not None:
+            ph = self.C["phase"]
+            keep = [j for j in range(len(self._slots))
+                    if ph[j] not in (DONE_OK, DONE_FAIL)]
+            for j in keep:
+                sl = self._slots[j]
+                for k in _CTRL:
+                    sl.ctrl[k] = self.C[k][j]
+                kept.append(sl)
+            if keep:
+                idx = np.concatenate([np.arange(self.off[j], self.off[j + 1])
+                                      for j in keep])
+                gather = (self.A, torch.as_tensor(idx, device=self.dev))
+        else:
+            kept = [sl for sl in self._slots
+                    if sl.ctrl["phase"] not in (DONE_OK, DONE_FAIL)]
+        self.batch = None
+        new = self._pending
+        self._slots = kept + new
         self._pending = []
         if not self._slots:
             return
@@ -192,10 +216,14 @@ class FlameletStepper:
         self.h_scale = enthalpy_scale(h_ox, h_fu)
         self.batch = b
         self.off = b.offsets_np
-        self.A = {k: torch.cat([sl.arrays[k] for sl in self._slots], dim=1)
-                  .contiguous() for k in _STATE}
-        self.A.update({k: torch.cat([sl.arrays[k] for sl in self._slots])
-                       for k in _POINT})
+        def packed(k, dim):
+            pieces = []
+            if gather is not None:
+                pieces.append(gather[0][k].index_select(dim, gather[1]))
+            pieces += [sl.arrays[k] for sl in new]
+            return torch.cat(pieces, dim=dim).contiguous()
+        self.A = {k: packed(k, 1) for k in _STATE}
+        self.A.update({k: packed(k, 0) for k in _POINT})
         self.C = {k: np.array([sl.ctrl[k] for sl in self._slots],
                               dtype=float if k == "err_prev" else np.int64)
                   for k in _CTRL}
