@@ -28,6 +28,8 @@ from __future__ import annotations
 
 from typing import List, Optional, Sequence
 
+import math
+
 import numpy as np
 import torch
 
@@ -46,30 +48,30 @@ def _ptr(t):
 def partition_table(sizes, part_len=None, alpha=None):
     """Cut systems of the given sizes into consecutive partitions of about
     ``part_len`` rows (default alpha sqrt(N) per system, at least 8, with
-    alpha = sqrt 2 unless given): returns numpy arrays pstart, pend, psys,
-    jpart, jbefore, jafter, roff (see ``pyro_flamelet_thomas_partitioned``)."""
-    import math
+    alpha = sqrt 2 unless given) of equal length up to one row: returns
+    numpy arrays pstart, pend, psys, jpart, jbefore, jafter, roff (see
+    ``pyro_flamelet_thomas_partitioned``)."""
     a = math.sqrt(2.0) if alpha is None else float(alpha)
-    pstart, pend, psys, jpart, jbefore, jafter = [], [], [], [], [], []
-    roff = [0]
-    o = 0
-    for s, n in enumerate(sizes):
-        m = part_len or max(8, int(round(a * math.sqrt(n))))
-        npart = max(1, -(-n // m))
-        first = len(pstart)
-        nj0 = roff[-1]
-        for q in range(npart):
-            pstart.append(o + (q * n) // npart)
-            pend.append(o + ((q + 1) * n) // npart)
-            psys.append(s)
-            jbefore.append(nj0 + q - 1 if q > 0 else -1)
-            jafter.append(nj0 + q if q < npart - 1 else -1)
-        for q in range(npart - 1):
-            jpart.append(first + q)
-        roff.append(nj0 + npart - 1)
-        o += n
-    i64 = lambda a: np.asarray(a, dtype=np.int64)
-    i32 = lambda a: np.asarray(a, dtype=np.int32)
+    n = np.asarray(sizes, dtype=np.int64)
+    if part_len:
+        m = np.full(n.shape, int(part_len), dtype=np.int64)
+    else:
+        m = np.maximum(8, np.round(a * np.sqrt(n)).astype(np.int64))
+    npart = np.maximum(1, -(-n // m))
+    offs = np.concatenate([[0], np.cumsum(n)])
+    first = np.concatenate([[0], np.cumsum(npart)])
+    psys = np.repeat(np.arange(n.size), npart)
+    q = np.arange(first[-1]) - first[psys]
+    nq, npq, o = n[psys], npart[psys], offs[psys]
+    pstart = o + (q * nq) // npq
+    pend = o + ((q + 1) * nq) // npq
+    roff = np.concatenate([[0], np.cumsum(npart - 1)])
+    inner = q < npq - 1
+    jbefore = np.where(q > 0, roff[psys] + q - 1, -1)
+    jafter = np.where(inner, roff[psys] + q, -1)
+    jpart = np.nonzero(inner)[0]
+    i64 = lambda v: np.asarray(v, dtype=np.int64)
+    i32 = lambda v: np.asarray(v, dtype=np.int32)
     return (i64(pstart), i64(pend), i32(psys), i32(jpart), i64(jbefore),
             i64(jafter), i64(roff))
 

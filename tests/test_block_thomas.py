@@ -226,3 +226,36 @@ def test_gpu_autotuner_schedule(device):
     finally:
         autotune.reset()
     assert autotune.schedule(10, device) == autotune.DEFAULT
+
+
+@pytest.mark.parametrize("part_len,alpha", [(None, None), (2, None),
+                                            (8, None), (None, 0.5),
+                                            (None, 3.0)])
+def test_partition_table_properties(part_len, alpha):
+    """Partitions cover every system contiguously in order, lengths within
+    a system differ by at most one row, and the interface tables agree:
+    interface j of a system lies between its partitions q and q+1."""
+    from pyrometheus_amd.flamelets.engine import partition_table
+    rng = np.random.default_rng(5)
+    sizes = [3, 4, 17, 64, 250, 1000, 5] + list(rng.integers(3, 2049, 40))
+    ps, pe, psys, jpart, jb, ja, roff = partition_table(sizes, part_len,
+                                                        alpha)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    assert len(roff) == len(sizes) + 1 and roff[0] == 0
+    for s, n in enumerate(sizes):
+        parts = np.nonzero(psys == s)[0]
+        assert parts[0] == roff[s] + s and parts[-1] == roff[s + 1] + s
+        assert ps[parts[0]] == offs[s] and pe[parts[-1]] == offs[s + 1]
+        assert np.array_equal(ps[parts[1:]], pe[parts[:-1]])
+        lens = pe[parts] - ps[parts]
+        assert lens.min() >= 1 and lens.max() - lens.min() <= 1
+        if part_len is None:
+            a = np.sqrt(2.0) if alpha is None else alpha
+            m = max(8, int(round(a * np.sqrt(n))))
+            assert len(parts) == max(1, -(-n // m))
+        nj = len(parts) - 1
+        assert roff[s + 1] - roff[s] == nj
+        for q, p in enumerate(parts):
+            assert jb[p] == (roff[s] + q - 1 if q > 0 else -1)
+            assert ja[p] == (roff[s] + q if q < nj else -1)
+        assert np.array_equal(jpart[roff[s]:roff[s + 1]], parts[:-1])
