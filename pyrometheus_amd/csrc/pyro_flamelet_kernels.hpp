@@ -19,6 +19,8 @@ namespace pyro_fl {
 using namespace pyro;
 constexpr int NS = mech::NS;
 constexpr int NV = NS + 1;
+// block rows of the assembly unrolled for small blocks only
+constexpr int kRowUnroll = NV <= 17 ? NV : 1;
 
 // Residual and block Jacobian of the steady (mode 0) or Crank-Nicolson
 // (mode 1) flamelet equations for a batch of flamelets whose points are
@@ -90,7 +92,7 @@ k_flamelet_assemble(const double* __restrict__ phi, int64_t ldp,
     rhs[i * NV + v] = -r;
   }
   double* blk = J + i * NV * NV;
-#pragma unroll(NV <= 17 ? NV : 1)
+#pragma unroll kRowUnroll
   for (int rr = 0; rr < NV; ++rr) {
 #pragma unroll
     for (int cc = 0; cc < NV; ++cc) {

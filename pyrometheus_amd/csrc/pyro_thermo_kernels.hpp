@@ -47,8 +47,10 @@ k_species_thermo(const double* __restrict__ T, int64_t t_stride,
   if constexpr (WHICH == kDCpR) mech::species_dcp_r(t, o);
   if constexpr (WHICH == kDHRT) mech::species_dh_rt(t, 1.0 / t, o);
   if constexpr (WHICH == kDSR) mech::species_ds_r(t, 1.0 / t, o);
+  // streaming stores: the outputs are not read again by this kernel
 #pragma unroll
-  for (int k = 0; k < mech::NS; ++k) out[k * ld + i] = o[k];
+  for (int k = 0; k < mech::NS; ++k)
+    __builtin_nontemporal_store(o[k], &out[k * ld + i]);
 }
 
 // ---------------------------------------------------------------------------
