@@ -98,6 +98,7 @@ class FlameletStepper:
                              "each other")
         self.graphs = bool(graphs)
         self._graph = None
+        self._inflight = None  # (event, live, eq, cn) between launch/complete
         self.graph_captures = 0
         # when a list is given, step() appends the (B, 2) update norms
         self.norm_log: Optional[list] = None
@@ -238,6 +239,10 @@ class FlameletStepper:
         self._ctl_dev = torch.zeros(4 * b.B, dtype=torch.int32,
                                     device=self.dev)
         b.dt = self._ctl_dev[2 * b.B:].view(torch.float64)
+        # the per-flamelet norms come back through pinned host memory
+        self._norm_pin = torch.zeros((b.B, 2), dtype=torch.float64,
+                                     pin_memory=pinned)
+        self._norm_np = self._norm_pin.numpy()
         self.sys = b.sys.long()
         for sl in self._slots:  # the packed arrays now own the state
             sl.arrays = {}
@@ -250,12 +255,20 @@ class FlameletStepper:
     def step(self):
         """Advance every live flamelet by one update; returns the list of
         ``(handle, FlameletResult)`` of the flamelets that finished."""
-        from .solver import FlameletResult, update_norm
-        from .state import _array_to_state
+        self.launch()
+        return self.complete()
+
+    def launch(self) -> bool:
+        """First half of ``step``: enqueue one update of every live flamelet
+        on the current stream and the copy of its norms to host memory,
+        without waiting; ``complete`` finishes the step.  Between the two
+        the host may serve other work (another batch's launch)."""
+        if self._inflight is not None:
+            raise RuntimeError("complete() the previous step first")
         if self._pending or self.batch is None:
             self._pack()
         if self.batch is None:
-            return []
+            return False
         s, b, A, C = self.s, self.batch, self.A, self.C
         phase = C["phase"]
         phi, tg, chi, chi0, vd, pres = (A["phi"], A["tg"], A["chi"],
@@ -289,7 +302,29 @@ class FlameletStepper:
             self._graph.replay()
         else:
             update()
-        sums = b.norm2.cpu().numpy()
+        self._norm_pin[:nb].copy_(b.norm2, non_blocking=True)
+        ev = torch.cuda.Event() if self.dev.type == "cuda" else None
+        if ev is not None:
+            ev.record(torch.cuda.current_stream(self.dev))
+        self._inflight = (ev, live, eq, cn)
+        return True
+
+    def complete(self):
+        """Second half of ``step``: wait for the launched update, run every
+        flamelet's control flow on its norms, retire the finished ones;
+        returns their ``(handle, FlameletResult)``."""
+        from .solver import FlameletResult, update_norm
+        from .state import _array_to_state
+        if self._inflight is None:
+            return []
+        ev, live, eq, cn = self._inflight
+        self._inflight = None
+        if ev is not None:
+            ev.synchronize()
+        s, b, A, C = self.s, self.batch, self.A, self.C
+        phase = C["phase"]
+        phi, tg, vd, pres = A["phi"], A["tg"], A["vd"], A["pres"]
+        sums = self._norm_np[:b.B].copy()
         if self.norm_log is not None:
             self.norm_log.append(sums.copy())
         delta = update_norm(sums, s.newton_norm, self.h_scale)

@@ -95,7 +95,9 @@ class ChemistryServer:
                                                   newton_norm="scaled")
         # both streams at the default priority
         self.stream_w = torch.cuda.Stream(self.dev)
-        self.stream_f = torch.cuda.Stream(self.dev)
+        # one stream per mechanism's flamelet batch (default priority), so
+        # the batches of different mechanisms overlap on the device
+        self.streams_f: Dict[str, torch.cuda.Stream] = {}
         self.steppers: Dict[str, FlameletStepper] = {}
         self.queue: collections.deque = collections.deque()
         self.wdot_inflight: List[Ticket] = []
@@ -173,7 +175,7 @@ class ChemistryServer:
                     st = FlameletStepper(self.gases[t.mechanism],
                                          self.settings, device=self.dev)
                     self.steppers[t.mechanism] = st
-                with torch.cuda.stream(self.stream_f):
+                with torch.cuda.stream(self._stream_f(t.mechanism)):
                     (h,) = st.add([req.problem])
                 self.flamelets[(t.mechanism, h)] = t
 
@@ -193,15 +195,29 @@ class ChemistryServer:
                 keep.append(t)
         self.wdot_inflight = keep
 
+    def _stream_f(self, mech: str) -> torch.cuda.Stream:
+        st = self.streams_f.get(mech)
+        if st is None:
+            st = self.streams_f[mech] = torch.cuda.Stream(self.dev)
+        return st
+
     def step(self):
         """Admit what fits, advance every flamelet batch by one iteration,
-        retire finished work."""
+        retire finished work.  Every batch's iteration is launched (on its
+        mechanism's stream) before any is waited for, so batches of
+        different mechanisms run at the same time."""
         self._admit()
+        launched = []
         for mech, st in self.steppers.items():
             if not st.n_live:
                 continue
-            with torch.cuda.stream(self.stream_f):
-                finished = st.step()
+            with torch.cuda.stream(self._stream_f(mech)):
+                if st.launch():
+                    launched.append((mech, st))
+        self.poll()
+        for mech, st in launched:
+            with torch.cuda.stream(self._stream_f(mech)):
+                finished = st.complete()
             for h, res in finished:
                 t = self.flamelets.pop((mech, h))
                 t.result = res
