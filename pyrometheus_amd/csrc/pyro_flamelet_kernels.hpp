@@ -19,8 +19,6 @@ namespace pyro_fl {
 using namespace pyro;
 constexpr int NS = mech::NS;
 constexpr int NV = NS + 1;
-// block rows of the assembly unrolled for small blocks only
-constexpr int kRowUnroll = NV <= 17 ? NV : 1;
 
 // Residual and block Jacobian of the steady (mode 0) or Crank-Nicolson
 // (mode 1) flamelet equations for a batch of flamelets whose points are
@@ -91,21 +89,42 @@ k_flamelet_assemble(const double* __restrict__ phi, int64_t ldp,
     }
     rhs[i * NV + v] = -r;
   }
-  double* blk = J + i * NV * NV;
-#pragma unroll kRowUnroll
-  for (int rr = 0; rr < NV; ++rr) {
-#pragma unroll
-    for (int cc = 0; cc < NV; ++cc) {
-      double val = mask * blk[rr * NV + cc];
-      if (rr == cc) val = fma(hc, bi, val);
-      if (mode == 1) val = (rr == cc ? 1.0 : 0.0) - hdt * val;
-      blk[rr * NV + cc] = val;
-    }
-  }
   const double lo = last ? 0.0 : 0.5 * chs[i + 1] * la[i + 1];
   const double up = last ? 0.0 : hc * ci;
   L[i] = (mode == 1) ? -hdt * lo : lo;
   U[i] = (mode == 1) ? -hdt * up : up;
+}
+
+// Central blocks in place, one thread per entry of J (P, NV, NV) so that
+// consecutive threads touch consecutive doubles: J = m J + hc b I, and
+// I - dt/2 (that) for Crank-Nicolson, with the point's scalars as in
+// k_flamelet_assemble.
+__global__ void __launch_bounds__(256)
+k_flamelet_blocks(double* __restrict__ J, const double* __restrict__ chi,
+                  const double* __restrict__ chi0,
+                  const double* __restrict__ lb, const int* __restrict__ sys,
+                  const int64_t* __restrict__ off,
+                  const int* __restrict__ modes,
+                  const double* __restrict__ dt, int64_t n) {
+  constexpr int64_t NB = static_cast<int64_t>(NV) * NV;
+  const int64_t f = cell_index();
+  if (f >= n * NB) return;
+  const int64_t i = f / NB;
+  const int e = static_cast<int>(f - i * NB);
+  const int rr = e / NV, cc = e - rr * NV;
+  const int s = sys[i];
+  const bool first = (i == off[s]);
+  const bool last = (i == off[s + 1] - 1);
+  const int mode0 = modes ? modes[s] : 0;
+  const double* __restrict__ chs = ((mode0 & 2) && chi0) ? chi0 : chi;
+  const double hc = 0.5 * chs[i];
+  const double mask = (first || last) ? 0.0 : 1.0;
+  const int mode = mode0 & 1;
+  const double hdt = (mode == 1) ? 0.5 * dt[s] : 0.0;
+  double val = mask * J[f];
+  if (rr == cc) val = fma(hc, lb[i], val);
+  if (mode == 1) val = (rr == cc ? 1.0 : 0.0) - hdt * val;
+  J[f] = val;
 }
 
 // phi += x for points of active systems (x is (P, NV) AoS) and, if tg is
@@ -180,6 +199,10 @@ PYRO_API int pyro_flamelet_assemble(
   pyro_fl::k_flamelet_assemble<<<pyro::grid_for(n, 256), 256, 0, stream>>>(
       phi, ldp, S, lds, J, chi, chi0, la, lb, lc, sys, off, bc, modes,
       phi_prev, r_prev, dt, rhs, L, U, r_out, n);
+  pyro_fl::k_flamelet_blocks<<<pyro::grid_for(
+                                   n * pyro_fl::NV * pyro_fl::NV, 256),
+                               256, 0, stream>>>(J, chi, chi0, lb, sys, off,
+                                                 modes, dt, n);
   return pyro::launch_status();
 }
 
