@@ -8,7 +8,9 @@
 // of elimination changes.
 //
 // Every system is cut into partitions [s, e) of a few dozen rows.  Each
-// partition q is reduced by two wavefronts at once:
+// partition q is reduced by two sweeps at once (for NV <= 16 four sweeps
+// share a wavefront, for larger NV each sweep is a workgroup of four
+// wavefronts):
 //   down: rows s..e-1 with the coupling to x_{s-1} carried as NV extra
 //         right-hand-side columns; gives x_{e-1} = a + B x_{s-1} + D x_e and
 //         stores [X_U | X_r | X_W] of every row for the recovery,
@@ -30,109 +32,6 @@
 namespace pyro_spike {
 
 using pyro_bt::bcast;
-
-// One sweep over a partition, one wavefront, lane = logical row of
-// [C' | slot | r' | E] (W = 3 NV + 1 doubles): slot holds the coupling to
-// the next row to be eliminated (the neighbour outside the partition at
-// the last one), E the coefficients of the outside unknown met first.
-template <int NV>
-__global__ void __launch_bounds__(64)
-k_spike_sweep(const double* __restrict__ L, const double* __restrict__ C,
-              const double* __restrict__ U, const double* __restrict__ R,
-              const int64_t* __restrict__ off,
-              const int64_t* __restrict__ pstart,
-              const int64_t* __restrict__ pend,
-              const int* __restrict__ psys, double* __restrict__ fact,
-              double* __restrict__ maps, int* __restrict__ n_singular) {
-  constexpr int W = 3 * NV + 1;
-  constexpr int MW = 2 * NV + 1;  // [a | B | D] per logical row
-  const int part = blockIdx.x >> 1;
-  const bool down = (blockIdx.x & 1) == 0;
-  const int lane = threadIdx.x;
-  const bool active = lane < NV;
-  const int r = active ? lane : 0;
-  const int64_t s = pstart[part], e = pend[part];
-  const int sy = psys[part];
-  const int64_t s0 = off[sy], s1 = off[sy + 1];
-  const int64_t m = e - s;
-  // the up map of a system's first partition is never used
-  if (!down && s == s0) return;
-  bool singular = false;
-  // row index of step t, its coupling to the next row of the sweep (the
-  // "slot"), and the next row's coupling back to it
-  auto row = [&](int64_t t) { return down ? s + t : e - 1 - t; };
-  auto slot_coef = [&](int64_t i) -> double {
-    if (down) return i < s1 - 1 ? U[i] : 0.0;
-    return i > s0 ? L[i - 1] : 0.0;
-  };
-  auto back_coef = [&](int64_t i) -> double {  // next row's coupling to row i
-    return down ? L[i] : U[i - 1];
-  };
-  double a[W];
-  int perm[NV];
-  int my_k = lane;
-  {
-    const int64_t i = row(0);
-#pragma unroll
-    for (int c = 0; c < NV; ++c) a[c] = C[(i * NV + r) * NV + c];
-    const double sc = slot_coef(i);
-#pragma unroll
-    for (int c = 0; c < NV; ++c) a[NV + c] = (c == r) ? sc : 0.0;
-    a[2 * NV] = R[i * NV + r];
-    // the outside unknown met first: x_{s-1} (down) or x_e (up)
-    double ec = 0.0;
-    if (down && s > s0) ec = -L[s - 1];
-    if (!down && e < s1) ec = -U[e - 1];
-#pragma unroll
-    for (int c = 0; c < NV; ++c) a[2 * NV + 1 + c] = (c == r) ? ec : 0.0;
-  }
-  for (int64_t t = 0; t < m; ++t) {
-    const int64_t i = row(t);
-    double cpre[NV], rpre = 0.0, bpre = 0.0, spre = 0.0;
-    if (t + 1 < m) {
-      const int64_t i1 = row(t + 1);
-#pragma unroll
-      for (int c = 0; c < NV; ++c) cpre[c] = C[(i1 * NV + r) * NV + c];
-      rpre = R[i1 * NV + r];
-      bpre = back_coef(i);
-      spre = slot_coef(i1);
-    }
-    pyro_bt::gauss_jordan_dpp<NV, W>(a, lane, my_k, perm, singular);
-    if (down && active) {
-      // [X_U | X_r | X_W] of logical row my_k
-      double* f = fact + (i * NV + my_k) * MW;
-#pragma unroll
-      for (int c = 0; c < MW; ++c) f[c] = a[NV + c];
-    }
-    if (t + 1 == m) break;
-    const int k = active ? my_k : 0;
-    const double l = bpre;
-#pragma unroll
-    for (int c = 0; c < NV; ++c)
-      a[c] = fma(-l, a[NV + c], __shfl(cpre[c], k));
-    a[2 * NV] = fma(-l, a[2 * NV], __shfl(rpre, k));
-#pragma unroll
-    for (int c = 0; c < NV; ++c) a[2 * NV + 1 + c] *= -l;
-#pragma unroll
-    for (int c = 0; c < NV; ++c) a[NV + c] = (c == k) ? spre : 0.0;
-  }
-  // map of the last row of the sweep: x = X_r + X_E x_first - X_slot x_next
-  //   down: x_{e-1} = X_r + X_W x_{s-1} - X_U x_e
-  //   up:   x_s     = X_r + X_V x_e     - X_L x_{s-1}
-  // stored as [a | B (coeff of x_{s-1}) | D (coeff of x_e)]
-  if (active) {
-    double* mp = maps + ((static_cast<int64_t>(part) * 2 + (down ? 0 : 1)) *
-                             NV + my_k) * MW;
-    mp[0] = a[2 * NV];
-#pragma unroll
-    for (int c = 0; c < NV; ++c) {
-      const double ext = a[2 * NV + 1 + c], nxt = -a[NV + c];
-      mp[1 + c] = down ? ext : nxt;
-      mp[1 + NV + c] = down ? nxt : ext;
-    }
-  }
-  if (singular && lane == 0 && n_singular) atomicAdd(n_singular, 1);
-}
 
 // Gauss-Jordan as pyro_bt::gauss_jordan_dpp for four independent systems
 // per wave, one per 16-lane row: lanes base..base+NV-1 (base = 16 g) hold
@@ -185,9 +84,9 @@ __device__ __forceinline__ void gauss_jordan_grp(double (&a)[W], int lane,
   }
 }
 
-// k_spike_sweep for NV <= 16 with four sweeps per wave: sweep task
+// The partition sweeps for NV <= 16, four sweeps per wave: sweep task
 // t = 4 blockIdx.x + lane / 16 is (partition t / 2, down when t is even),
-// the same work as blockIdx t of k_spike_sweep.  Consecutive tasks are the
+// the same work as blockIdx t of k_spike_sweep_waves.  Consecutive tasks are the
 // two sweeps of neighbouring partitions of one system, so the four sweeps
 // of a wave have about the same length.
 template <int NV>
@@ -473,6 +372,406 @@ k_spike_recover(const double* __restrict__ fact,
   }
 }
 
+// ---- NV > 16: one workgroup of four wavefronts per sweep or system ----
+// The lanes hold the rows (lanes < NV), and every NV-wide block of columns
+// is spread over the four waves by column index: wave w holds the columns
+// c = 4 t + w (t < kT) of each block; columns c >= NV hold zeros.  One
+// extra column ("x", a right-hand side) is carried by wave 0 only; the other
+// waves carry zeros there.
+constexpr int kWaves = 4;
+
+template <int NV>
+struct Tiles {
+  static constexpr int T = (NV + kWaves - 1) / kWaves;  // columns per wave
+  static constexpr int LD = kWaves * T + 1;  // odd LDS row stride (doubles)
+  static constexpr int XC = kWaves * T;      // LDS column of "x"
+};
+
+struct GjShared {
+  double g[2][64];       // multipliers of pivot step k at [k & 1]
+  double smax[kWaves][64];
+  int p[2];              // pivot lane of step k at [k & 1]
+  int singular;
+};
+
+// Gauss-Jordan with scaled partial pivoting as pyro_bt::gauss_jordan_dpp,
+// for rows held one per lane and columns spread over the four waves of a
+// workgroup.  a0 is the block that holds the pivot columns, ab[NB] further
+// blocks, x the extra column.  Pivot step k: the wave owning column k
+// (k mod 4) finds the pivot lane p by a DPP max-reduction of the scaled
+// |a0[k]| over the unused rows and writes the multipliers g (1 - 1/piv on
+// the pivot row, a[k]/piv elsewhere) and p to LDS; after one barrier every
+// wave updates its columns c > k (and a few dead columns c <= k, whose
+// values are never read again) with one FMA each, the pivot row's entries
+// read from lane p by v_readlane.  Double-buffered multipliers need one
+// barrier per step.  On return lane l holds logical row my_k of [I | X].
+template <int NV, int T, int NB>
+__device__ __forceinline__ void gj_waves(double (&a0)[T], double (&ab)[NB][T],
+                                         double& x, int lane, int wv,
+                                         int& my_k, GjShared& sh) {
+  static_assert(NV <= 64, "one wavefront holds the rows");
+  constexpr int kRows = (NV + 15) / 16;
+  const bool active = lane < NV;
+  {
+    double mx = 0.0;
+#pragma unroll
+    for (int t = 0; t < T; ++t) mx = fmax(mx, fabs(a0[t]));
+    sh.smax[wv][lane] = mx;
+    __syncthreads();
+    mx = fmax(fmax(sh.smax[0][lane], sh.smax[1][lane]),
+              fmax(sh.smax[2][lane], sh.smax[3][lane]));
+    const double sc = mx > 0.0 ? pyro::recip(mx) : 1.0;
+#pragma unroll
+    for (int t = 0; t < T; ++t) a0[t] *= sc;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int t = 0; t < T; ++t) ab[b][t] *= sc;
+    x *= sc;
+  }
+  unsigned long long used = 0ull;
+  const unsigned long long all = NV == 64 ? ~0ull : ((1ull << NV) - 1ull);
+  my_k = -1;
+  bool singular = false;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const int tk = k / kWaves;
+    if (wv == k % kWaves) {
+      double key = -1.0;
+      if (active && !((used >> lane) & 1ull)) {
+        const double v = fabs(a0[tk]);
+        key = (v == v) ? v : -0.5;  // a NaN row only as a last resort
+      }
+      long long kb = __double_as_longlong(key);
+      if (key >= 0.0) kb = (kb & ~0x3Fll) | (63 - lane);
+      const double rm = pyro_bt::row16_max(__longlong_as_double(kb));
+      double best = bcast(rm, 0);
+#pragma unroll
+      for (int q = 1; q < kRows; ++q) best = fmax(best, bcast(rm, 16 * q));
+      int p;
+      if (best >= 0.0)
+        p = 63 - static_cast<int>(__double_as_longlong(best) & 0x3F);
+      else
+        p = __builtin_ctzll(~used & all);
+      p = __builtin_amdgcn_readfirstlane(p);
+      const double piv = bcast(a0[tk], p);
+      singular = singular || !(fabs(piv) > 0.0);
+      const double inv = pyro::recip(piv);
+      sh.g[k & 1][lane] = (lane == p) ? 1.0 - inv : a0[tk] * inv;
+      if (lane == 0) sh.p[k & 1] = p;
+    }
+    __syncthreads();
+    const int p = __builtin_amdgcn_readfirstlane(sh.p[k & 1]);
+    const double g = sh.g[k & 1][lane];
+    used |= 1ull << p;
+    if (lane == p) my_k = k;
+#pragma unroll
+    for (int t = tk; t < T; ++t) a0[t] = fma(-g, bcast(a0[t], p), a0[t]);
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int t = 0; t < T; ++t) ab[b][t] = fma(-g, bcast(ab[b][t], p), ab[b][t]);
+    x = fma(-g, bcast(x, p), x);
+  }
+  if (singular && lane == 0) sh.singular = 1;
+}
+
+// One sweep over a partition for NV > 16, one workgroup of four waves per
+// sweep task (blockIdx = 2 partition + (0 down, 1 up)).  Lane = logical row
+// of [C' | slot | r' | E], spread over the waves (a0 = C', ab[0] = slot,
+// ab[1] = E, x = r'): slot holds the coupling to the next row to be
+// eliminated (the neighbour outside the partition at the last one), E the
+// coefficients of the outside unknown met first (x_{s-1} down, x_e up).
+// After each row's Gauss-Jordan the down sweep stores [X_U | X_r | X_W] of
+// every row for the recovery; the last row gives the partition's map
+//   down: x_{e-1} = X_r + X_W x_{s-1} - X_U x_e
+//   up:   x_s     = X_r + X_V x_e     - X_L x_{s-1},
+// stored as [a | B (coefficient of x_{s-1}) | D (coefficient of x_e)].
+// k_spike_sweep_grp does the same for NV <= 16.
+template <int NV>
+__global__ void __launch_bounds__(256)
+k_spike_sweep_waves(const double* __restrict__ L, const double* __restrict__ C,
+                    const double* __restrict__ U, const double* __restrict__ R,
+                    const int64_t* __restrict__ off,
+                    const int64_t* __restrict__ pstart,
+                    const int64_t* __restrict__ pend,
+                    const int* __restrict__ psys, double* __restrict__ fact,
+                    double* __restrict__ maps, int* __restrict__ n_singular) {
+  constexpr int T = Tiles<NV>::T;
+  constexpr int MW = 2 * NV + 1;
+  __shared__ GjShared sh;
+  const int64_t part = blockIdx.x >> 1;
+  const bool down = (blockIdx.x & 1) == 0;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool active = lane < NV;
+  const int r = active ? lane : 0;
+  const int64_t s = pstart[part], e = pend[part];
+  const int sy = psys[part];
+  const int64_t s0 = off[sy], s1 = off[sy + 1];
+  const int64_t m = e - s;
+  if (!down && s == s0) return;  // never used; the whole workgroup leaves
+  if (threadIdx.x == 0) sh.singular = 0;
+  auto row = [&](int64_t t) { return down ? s + t : e - 1 - t; };
+  auto slot_coef = [&](int64_t i) -> double {
+    if (down) return i < s1 - 1 ? U[i] : 0.0;
+    return i > s0 ? L[i - 1] : 0.0;
+  };
+  auto back_coef = [&](int64_t i) -> double {
+    return down ? L[i] : U[i - 1];
+  };
+  auto col = [&](int t) { return kWaves * t + wv; };
+  double a0[T], ab[2][T], x;
+  int my_k = lane;
+  {
+    const int64_t i = row(0);
+    const double sc = slot_coef(i);
+    double ec = 0.0;
+    if (down && s > s0) ec = -L[s - 1];
+    if (!down && e < s1) ec = -U[e - 1];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const int c = col(t);
+      a0[t] = c < NV ? C[(i * NV + r) * NV + c] : 0.0;
+      ab[0][t] = (c == r) ? sc : 0.0;
+      ab[1][t] = (c == r) ? ec : 0.0;
+    }
+    x = wv == 0 ? R[i * NV + r] : 0.0;
+  }
+  for (int64_t t = 0; t < m; ++t) {
+    const int64_t i = row(t);
+    double cpre[T], rpre = 0.0, bpre = 0.0, spre = 0.0;
+    if (t + 1 < m) {
+      const int64_t i1 = row(t + 1);
+#pragma unroll
+      for (int u = 0; u < T; ++u) {
+        const int c = col(u);
+        cpre[u] = c < NV ? C[(i1 * NV + r) * NV + c] : 0.0;
+      }
+      if (wv == 0) rpre = R[i1 * NV + r];
+      bpre = back_coef(i);
+      spre = slot_coef(i1);
+    }
+    gj_waves<NV, T, 2>(a0, ab, x, lane, wv, my_k, sh);
+    if (down && active) {
+      // [X_U | X_r | X_W] of logical row my_k
+      double* f = fact + (i * NV + my_k) * MW;
+#pragma unroll
+      for (int u = 0; u < T; ++u) {
+        const int c = col(u);
+        if (c < NV) {
+          f[c] = ab[0][u];
+          f[NV + 1 + c] = ab[1][u];
+        }
+      }
+      if (wv == 0) f[NV] = x;
+    }
+    if (t + 1 == m) break;
+    const int k = active ? my_k : 0;
+    const double l = bpre;
+#pragma unroll
+    for (int u = 0; u < T; ++u) {
+      a0[u] = fma(-l, ab[0][u], __shfl(cpre[u], k));
+      ab[1][u] *= -l;
+      ab[0][u] = (col(u) == k) ? spre : 0.0;
+    }
+    x = fma(-l, x, __shfl(rpre, k));
+  }
+  if (active) {
+    double* mp = maps + ((part * 2 + (down ? 0 : 1)) * NV + my_k) * MW;
+    if (wv == 0) mp[0] = x;
+#pragma unroll
+    for (int u = 0; u < T; ++u) {
+      const int c = col(u);
+      if (c < NV) {
+        const double ext = ab[1][u], nxt = -ab[0][u];
+        mp[1 + c] = down ? ext : nxt;
+        mp[1 + NV + c] = down ? nxt : ext;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && sh.singular && n_singular) atomicAdd(n_singular, 1);
+}
+
+// k_spike_reduced for NV > 16: one workgroup of four waves per system.
+// G_{q-1} (with beta_{q-1} in column XC), P = B G_{q-1} (with
+// a + B beta_{q-1} in column XC) and H_q (with tau_q in column XC) live in
+// LDS, row-major with an odd stride; the products read one operand as an
+// LDS broadcast, the other from the lane's own row.  Per partition:
+//   A: lane r, wave w: row r of [I - B' G | a' + B' beta | D'] in the
+//      wave's columns, and row r of P to LDS;
+//   B: Gauss-Jordan over the waves (gj_waves) gives [tau | H], stored by
+//      logical row to LDS and to red;
+//   C: row r of G_q = D + P H and beta_q = a + B beta + P tau, to LDS and
+//      red.
+// The back substitution is k_spike_reduced's, on wave 0.
+template <int NV>
+__global__ void __launch_bounds__(256)
+k_spike_reduced_waves(const double* __restrict__ maps,
+                      const int64_t* __restrict__ roff,
+                      double* __restrict__ red, double* __restrict__ Z,
+                      int* __restrict__ n_singular) {
+  constexpr int T = Tiles<NV>::T;
+  constexpr int LD = Tiles<NV>::LD;
+  constexpr int XC = Tiles<NV>::XC;
+  constexpr int MW = 2 * NV + 1;
+  constexpr int N2 = 2 * NV;
+  constexpr int RW = 2 * NV + 2;
+  __shared__ double Gs[NV * LD], Ps[NV * LD], Hs[NV * LD];
+  __shared__ GjShared sh;
+  const int sy = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool active = lane < NV;
+  const int r = active ? lane : 0;
+  const int64_t j0 = roff[sy], j1 = roff[sy + 1];
+  const int64_t nq = j1 - j0 + 1;
+  if (nq < 2) return;  // the whole workgroup leaves
+  const int64_t q0 = j0 + sy;
+  auto mrow = [&](int64_t q, int dir) {
+    return maps + ((q * 2 + dir) * NV + r) * MW;
+  };
+  auto rrow = [&](int64_t q, int row) { return red + (q * NV + row) * RW; };
+  auto col = [&](int t) { return kWaves * t + wv; };
+  // the padding columns NV..XC-1 of G and H stay zero
+  for (int idx = threadIdx.x; idx < NV * LD; idx += 256) {
+    Gs[idx] = 0.0;
+    Hs[idx] = 0.0;
+  }
+  if (threadIdx.x == 0) sh.singular = 0;
+  __syncthreads();
+  {
+    const double* md = mrow(q0, 0);
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int c = col(t);
+        if (c < NV) {
+          const double v = md[1 + NV + c];
+          Gs[r * LD + c] = v;
+          rrow(q0, r)[1 + c] = v;
+        }
+      }
+      if (wv == 0) {
+        Gs[r * LD + XC] = md[0];
+        rrow(q0, r)[0] = md[0];
+      }
+    }
+  }
+  __syncthreads();
+  for (int64_t q = q0 + 1; q < q0 + nq; ++q) {
+    const double* mu = mrow(q, 1);
+    const double* md = mrow(q, 0);
+    double a0[T], ab[1][T], pp[T];
+    double x = mu[0], bs = md[0];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      a0[t] = (col(t) == r) ? 1.0 : 0.0;
+      pp[t] = 0.0;
+    }
+#pragma unroll 6
+    for (int mm = 0; mm < NV; ++mm) {
+      const double bp = mu[1 + mm], bm = md[1 + mm];
+      const double* gm = Gs + mm * LD;
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const double gv = gm[col(t)];
+        a0[t] = fma(-bp, gv, a0[t]);
+        pp[t] = fma(bm, gv, pp[t]);
+      }
+      const double gb = gm[XC];
+      x = fma(bp, gb, x);
+      bs = fma(bm, gb, bs);
+    }
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < T; ++t)
+        if (col(t) < NV) Ps[r * LD + col(t)] = pp[t];
+      if (wv == 0) Ps[r * LD + XC] = bs;
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const int c = col(t);
+      ab[0][t] = (active && c < NV) ? mu[1 + NV + c] : 0.0;
+      if (!active) a0[t] = 0.0;
+    }
+    if (wv != 0 || !active) x = 0.0;
+    int my_k;
+    gj_waves<NV, T, 1>(a0, ab, x, lane, wv, my_k, sh);
+    if (active) {
+      double* o = rrow(q, my_k);
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int c = col(t);
+        if (c < NV) {
+          Hs[my_k * LD + c] = ab[0][t];
+          o[2 + NV + c] = ab[0][t];
+        }
+      }
+      if (wv == 0) {
+        Hs[my_k * LD + XC] = x;
+        o[1 + NV] = x;
+      }
+    }
+    __syncthreads();
+    double gq[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const int c = col(t);
+      gq[t] = c < NV ? md[1 + NV + c] : 0.0;
+    }
+    double bq = Ps[r * LD + XC];
+#pragma unroll 6
+    for (int mm = 0; mm < NV; ++mm) {
+      const double pm = Ps[r * LD + mm];
+      const double* hm = Hs + mm * LD;
+#pragma unroll
+      for (int t = 0; t < T; ++t) gq[t] = fma(pm, hm[col(t)], gq[t]);
+      bq = fma(pm, hm[XC], bq);
+    }
+    if (active) {
+      double* o = rrow(q, r);
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int c = col(t);
+        if (c < NV) {
+          Gs[r * LD + c] = gq[t];
+          o[1 + c] = gq[t];
+        }
+      }
+      if (wv == 0) {
+        Gs[r * LD + XC] = bq;
+        o[0] = bq;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && sh.singular && n_singular) atomicAdd(n_singular, 1);
+  if (wv != 0) return;
+  // back substitution on wave 0 (the rows of red were written by all four
+  // waves before the last barrier); t in natural row order, lane r
+  const int64_t ql = q0 + nq - 1;
+  double t = rrow(ql, r)[1 + NV];
+  for (int64_t q = ql - 1; q >= q0; --q) {
+    const double* o = rrow(q, r);
+    double b = o[0];
+#pragma unroll
+    for (int mm = 0; mm < NV; ++mm) b = fma(o[1 + mm], bcast(t, mm), b);
+    const int64_t j = j0 + (q - q0);
+    if (active) {
+      Z[j * N2 + r] = b;
+      Z[j * N2 + NV + r] = t;
+    }
+    if (q > q0) {
+      double tn = o[1 + NV];
+#pragma unroll
+      for (int mm = 0; mm < NV; ++mm) tn = fma(o[2 + NV + mm], bcast(t, mm), tn);
+      t = tn;
+    }
+  }
+}
+
 // Host side: the three launches.  All arrays are device arrays prepared by
 // the caller (see pyro_flamelet_thomas_partitioned).
 template <int NV>
@@ -488,12 +787,19 @@ int solve(const double* L, const double* C, const double* U, const double* R,
             L, C, U, R, off, pstart, pend, psys, nparts, fact, maps,
             n_singular);
   } else {
-    k_spike_sweep<NV><<<static_cast<unsigned>(2 * nparts), 64, 0, stream>>>(
-        L, C, U, R, off, pstart, pend, psys, fact, maps, n_singular);
+    k_spike_sweep_waves<NV>
+        <<<static_cast<unsigned>(2 * nparts), 64 * kWaves, 0, stream>>>(
+            L, C, U, R, off, pstart, pend, psys, fact, maps, n_singular);
   }
-  if (nj > 0)
-    k_spike_reduced<NV><<<static_cast<unsigned>(nsys), 64, 0, stream>>>(
-        maps, roff, red, Z, n_singular);
+  if (nj > 0) {
+    if constexpr (NV <= 16)
+      k_spike_reduced<NV><<<static_cast<unsigned>(nsys), 64, 0, stream>>>(
+          maps, roff, red, Z, n_singular);
+    else
+      k_spike_reduced_waves<NV>
+          <<<static_cast<unsigned>(nsys), 64 * kWaves, 0, stream>>>(
+              maps, roff, red, Z, n_singular);
+  }
   k_spike_recover<NV><<<static_cast<unsigned>(nparts), 64, 0, stream>>>(
       fact, pstart, pend, jbefore, jafter, Z, X);
   return pyro::launch_status();

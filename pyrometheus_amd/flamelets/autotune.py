@@ -3,11 +3,11 @@
 A schedule is either ``"sequential"`` -- one wavefront walks each flamelet's
 block rows (``k_block_thomas``) -- or ``"partitioned"`` with a flamelet of N
 points cut into partitions of max(8, round(alpha sqrt(N))) rows, each swept
-from both ends by two wavefronts, then the interface system and the
+from both ends by two sweeps, then the nv-sized interface recursion and the
 recovery (``pyro_flamelet_thomas_partitioned``).  Which is fastest, and with
 which alpha, depends on the block size, on the device and on the spread of
-flamelet sizes: short partitions mean short chains but a longer interface
-system with 2nv x 2nv blocks.
+flamelet sizes: short partitions mean short sweeps but a longer interface
+recursion.
 
 The schedule is kept per (device, nv), never per batch: a flamelet's
 partitioning then depends on its own size only, so its arithmetic -- and its

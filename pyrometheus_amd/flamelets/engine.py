@@ -15,9 +15,9 @@ launch sequence:
    steady or Crank-Nicolson per flamelet;
 3. ``pyro_flamelet_thomas_partitioned`` -- block Thomas with the Laplacian
    coupling blocks as scalar multiples of the identity, each flamelet cut
-   into partitions swept by two wavefronts each and joined through a small
-   interface system (``pyro_flamelet_thomas``, one wavefront per flamelet,
-   for nv > 32 or when asked);
+   into partitions swept from both ends and joined through an nv-sized
+   interface recursion, nv <= 64 (``pyro_flamelet_thomas``, one wavefront
+   per flamelet, for larger nv or when asked);
 4. ``pyro_flamelet_update`` -- phi += v for the selected flamelets and the
    per-flamelet sums of squares of v over the enthalpy row and over the
    species rows (fixed-order reductions, so a flamelet's result does not
