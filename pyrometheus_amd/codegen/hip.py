@@ -121,6 +121,15 @@ class _Writer:
 #:           q = k_f prod_r F_r (prod_r Z_r - (RT/p0)^dnu prod_p Z_p): 53
 #:           reciprocals instead of one per reversible reaction (309 for
 #:           GRI-3.0), 4 % fewer VALU instructions, 6 % faster.
+#:  jac_groups -- waves of a k_source_jacobian_rows block that split the
+#:           rows of the same cells (1; 2 and 4, with one or two waves per
+#:           SIMD, measured 14-25 % slower than 1: the row accumulators,
+#:           the concentrations and the hoisted cache loads outgrow 256
+#:           registers);
+#:  jac_waves -- waves per SIMD the compiler must budget registers for in
+#:           k_source_jacobian_rows (0: its own choice);
+#:  jac_fence -- an empty asm with a memory clobber every that many
+#:           reactions of a source-Jacobian row (0: none).
 #: (GRI-3.0, 2^24 cells: 16.6 ms with the species arrays held in registers,
 #: 12.1 ms lazy at two waves per SIMD, 11.35 ms with zform.)
 DEFAULT_TUNING = {"pin": True, "pin_every": 3, "kc": "species", "ctable": True,
@@ -586,6 +595,8 @@ def _emit_prelude(w: _Writer, L: _Lowering, name: str, digest: str,
     w(f"constexpr int kWdotWaves = {int(L.tuning.get('wdot_waves', 0))};")
     w(f"constexpr int kDerivWaves = {int(L.tuning.get('deriv_waves', 0))};")
     w(f"constexpr int kTempWaves = {int(L.tuning.get('temp_waves', 0))};")
+    w(f"constexpr int kJacWaves = {int(L.tuning.get('jac_waves', 0))};")
+    w(f"constexpr int kJacGroups = {int(L.tuning.get('jac_groups', 1))};")
     w("__device__ constexpr double kW[NS] = {"
       + ", ".join(lit(x) for x in m.molecular_weights) + "};")
     w("__device__ constexpr double kInvW[NS] = {"
@@ -1561,17 +1572,38 @@ def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
       "double (&C)[NS], const double rho, const double T, "
       "const double cp, const double mmw, const double vd, "
       "const double* __restrict__ work, const unsigned il, "
-      "const double* dTl, const int hs, "
+      "pyro::lptr dTl, const int hs, "
       "double* __restrict__ o, const int64_t s_rc, "
-      "double* __restrict__ S, const int64_t ldS) {")
+      "double* __restrict__ S, const int64_t ldS, const int rg) {")
     w.ind += 1
     w("pyro::cptr IW = (pyro::cptr)kInvWMem;")
+    # the cache as a global-memory pointer: loads from a uniform base plus
+    # the lane's 32-bit offset rather than a 64-bit address per load
+    w("pyro::gptr wg = (pyro::gptr)work;")
     w("double irho = 1.0 / rho;")
     w("double rT = rho * (1.0 / T);")
     w("double dT0 = 1.0 / cp;")
     w("double dr0 = -rT * dT0;")
     w("double rm = -rho * mmw;  // drho_m = -rT dT_m + rm / W_m")
     w(f"constexpr int NV = {ns + 1};")
+    # rows 1..ns split into kJacGroups groups of about equal work (longest
+    # first onto the lightest group); row 0 goes with group 0.  Group rg is
+    # one wave-uniform branch.
+    inv = {k: [] for k in range(ns)}
+    for i, r in enumerate(m.reactions):
+        for k, nu in L.species_updates(r, "q"):
+            inv[k].append((i, nu))
+    ng = int(L.tuning.get("jac_groups", 1))
+    rows = list(range(L.tuning.get("jac_rows", ns)))
+    load = [0.0] * ng
+    load[0] = ns
+    group_of = {}
+    for k in sorted(rows, key=lambda k: -(len(inv[k]) * 8 + ns)):
+        g = min(range(ng), key=lambda g: load[g])
+        group_of[k] = g
+        load[g] += len(inv[k]) * 8 + ns
+    w("if (rg == 0) {")
+    w.ind += 1
     w("{  // row 0: d(vd/rho)")
     w.ind += 1
     w("const double v0 = -vd * irho * irho;")
@@ -1584,12 +1616,13 @@ def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
     w("if (S) S[0] = vd * irho;")
     w.ind -= 1
     w("}")
-    # reactions involving each species, with the net coefficient
-    inv = {k: [] for k in range(ns)}
-    for i, r in enumerate(m.reactions):
-        for k, nu in L.species_updates(r, "q"):
-            inv[k].append((i, nu))
-    for k in range(L.tuning.get("jac_rows", ns)):
+    cur = 0
+    for k in sorted(rows, key=lambda k: (group_of[k], k)):
+        if group_of[k] != cur:
+            cur = group_of[k]
+            w.ind -= 1
+            w(f"}} else if (rg == {cur}) {{")
+            w.ind += 1
         w(f"{{  // row {k + 1}: {m.species[k].name}")
         w.ind += 1
         # fresh copies of C and of the per-cell scalars for this row:
@@ -1602,18 +1635,22 @@ def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
           '"+v"(irho));')
         # fresh copies of the cache and LDS pointers: this row's loads cannot
         # be hoisted above the previous row (and kept live meanwhile)
-        w('asm volatile("" : "+s"(work), "+v"(dTl), "+s"(IW));')
+        w('asm volatile("" : "+s"(wg), "+v"(dTl), "+s"(IW));')
         w("double G[NS];")
         w("#pragma unroll")
         w("for (int j = 0; j < NS; ++j) G[j] = 0.0;")
         w("double wk = 0.0, bk = 0.0, dm = 0.0;")
-        for i, nu in inv[k]:
+        jf = int(L.tuning.get("jac_fence", 0))
+        for n_r, (i, nu) in enumerate(inv[k]):
+            if jf and n_r and n_r % jf == 0:
+                # at most jac_fence reactions' loads in flight
+                w('asm volatile("" ::: "memory");')
             r = m.reactions[i]
             nl = lit(nu)
             w(f"{{  // R{i}: {r.equation}")
             w.ind += 1
             def at(nm, i=i):
-                return f"(work + {slots[(i, nm)]} * kCacheLd)[il]"
+                return f"(wg + {slots[(i, nm)]} * kCacheLd)[il]"
             w(f"const double kf = {at('kf')};")
             w(f"wk = fma({nl}, {at('q')}, wk);")
             w(f"bk = fma({nl}, {at('dqT')}, bk);")
@@ -1661,6 +1698,8 @@ def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
         w('asm volatile("" ::: "memory");')
         w.ind -= 1
         w("}")
+    w.ind -= 1
+    w("}")
     w.ind -= 1
     w("}")
     w("")

@@ -57,6 +57,8 @@ typedef const __attribute__((address_space(4))) double* cptr;
 // Pointer into global memory (keeps loads through a pointer that passed an
 // asm statement on the global path rather than the flat one).
 typedef const __attribute__((address_space(1))) double* gptr;
+// Pointer into LDS (loads through it stay ds_read rather than flat).
+typedef const __attribute__((address_space(3))) double* lptr;
 
 // Reciprocal from the hardware estimate (relative error below 5e-8 on
 // gfx950, measured) and one Newton step: relative error below 2.3e-15.

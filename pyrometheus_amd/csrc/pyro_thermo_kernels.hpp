@@ -266,7 +266,13 @@ k_reaction_cache(const double* __restrict__ T, const double* __restrict__ p,
   mech::reaction_cache(t, C, work, static_cast<unsigned>(i));
 }
 
-__global__ void __launch_bounds__(kBlock)
+// Cells per block of k_source_jacobian_rows: each of the block's
+// mech::kJacGroups waves builds one group of rows (balanced by work) for
+// the same kJacCells cells, so the dT table in LDS is shared by the groups.
+constexpr int kJacCells = kBlock / mech::kJacGroups;
+static_assert(kJacCells % 64 == 0, "a row group is whole waves");
+
+__global__ void __launch_bounds__(kBlock, mech::kJacWaves > 0 ? mech::kJacWaves : 1)
 k_source_jacobian_rows(const double* __restrict__ T,
                        const double* __restrict__ p, int64_t p_stride,
                        const double* __restrict__ Y, int64_t ldY,
@@ -275,9 +281,13 @@ k_source_jacobian_rows(const double* __restrict__ T,
                        double* __restrict__ out, int64_t s_rc, int64_t s_i,
                        double* __restrict__ S, int64_t ldS, int64_t n) {
   // dT of column 1+m for every cell of the block, read by every row
-  __shared__ double dTl[mech::NS * kBlock];
-  const int64_t i = cell_index();
-  if (i >= n) return;
+  __shared__ double dTl[mech::NS * kJacCells];
+  const int cl = threadIdx.x % kJacCells;
+  const int rg = __builtin_amdgcn_readfirstlane(threadIdx.x / kJacCells);
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kJacCells + cl;
+  // lanes past the end work on the last cell (no early return before the
+  // barrier) and store nothing
+  const int64_t i = i0 < n ? i0 : n - 1;
   double t, rho, mmw, C[mech::NS];
   source_state(T, p, p_stride, Y, ldY, i, t, rho, mmw, C);
   // one species array live at a time: cp first, then h/RT straight to LDS
@@ -295,13 +305,19 @@ k_source_jacobian_rows(const double* __restrict__ T,
     double hrt[mech::NS];
     mech::species_h_rt(t, 1.0 / t, hrt);
     const double kT = -kGasConstant * t / cp;
+    // the groups write the species k = rg (mod groups)
 #pragma unroll
     for (int k = 0; k < mech::NS; ++k)
-      dTl[k * kBlock + threadIdx.x] = kT * hrt[k] * mech::kInvW[k];
+      if (k % mech::kJacGroups == rg)
+        dTl[k * kJacCells + cl] = kT * hrt[k] * mech::kInvW[k];
   }
+  __syncthreads();
+  if (i0 >= n) return;
   mech::source_jacobian_rows(C, rho, t, cp, mmw, vd ? vd[i] : 0.0, work,
-                             static_cast<unsigned>(i), dTl + threadIdx.x, kBlock,
-                             out + i * s_i, s_rc, S ? S + i : nullptr, ldS);
+                             static_cast<unsigned>(i),
+                             (pyro::lptr)(dTl + cl), kJacCells,
+                             out + i * s_i, s_rc, S ? S + i : nullptr, ldS,
+                             rg);
 }
 
 // ---------------------------------------------------------------------------
@@ -542,7 +558,7 @@ PYRO_API int pyro_source_jacobian(const double* T, const double* p,
     const int64_t nc = n - c0 < chunk ? n - c0 : chunk;
     k_reaction_cache<<<pyro::grid_for(nc, kBlock), kBlock, 0, stream>>>(
         T + c0, p + c0 * p_stride, p_stride, Y + c0, ldY, work, nc);
-    k_source_jacobian_rows<<<pyro::grid_for(nc, kBlock), kBlock, 0,
+    k_source_jacobian_rows<<<pyro::grid_for(nc, kJacCells), kBlock, 0,
                              stream>>>(
         T + c0, p + c0 * p_stride, p_stride, Y + c0, ldY,
         vd ? vd + c0 : nullptr, work, out + c0 * s_i, s_rc, s_i,
