@@ -121,6 +121,11 @@ class _Writer:
 #:           q = k_f prod_r F_r (prod_r Z_r - (RT/p0)^dnu prod_p Z_p): 53
 #:           reciprocals instead of one per reversible reaction (309 for
 #:           GRI-3.0), 4 % fewer VALU instructions, 6 % faster.
+#:  kinv_f -- the lazy JVP / VJP passes keep F_k = 1 / E_k per open species
+#:           and take K_c^-1 as a product of E (products) and F (reactants)
+#:           without a reciprocal per reaction (off: the extra live values
+#:           cost more than the reciprocals, JVP 9.9 -> 10.3 ms and VJP
+#:           8.7 -> 9.0 ms on 2^22 GRI-3.0 cells);
 #:  jac_groups -- waves of a k_source_jacobian_rows block that split the
 #:           rows of the same cells (1; 2 and 4, with one or two waves per
 #:           SIMD, measured 14-25 % slower than 1: the row accumulators,
@@ -195,6 +200,9 @@ class _Lowering:
         self.fexp = "exp"
         # reciprocal in K_c^-1 (products of Gibbs factors)
         self.frecip = "pyro::recip"
+        # K_c^-1 from E[] of the products and F[] = 1 / E[] of the
+        # reactants (set while emitting the lazy derivative passes)
+        self.kinv_f = False
 
     _INLINE = {0.0, 0.5, -0.5, 1.0, -1.0, 2.0, -2.0, 4.0, -4.0}
 
@@ -320,15 +328,21 @@ class _Lowering:
               f" }}  // {sp.name}")
 
     def kinv_expr(self, r: Reaction) -> str:
-        """exp(-ln K_c) from the E[] factors (balanced, integer nu)."""
-        def side(d, extra):
+        """exp(-ln K_c) from the E[] factors (balanced, integer nu); with
+        ``kinv_f`` the reactants' factors come from F[] = 1 / E[] and no
+        reciprocal is taken per reaction."""
+        def side(d, extra, arr="E"):
             ts = []
             for sp, nu in d:
-                ts += [f"E[{self.idx[sp]}]"] * int(nu)
+                ts += [f"{arr}[{self.idx[sp]}]"] * int(nu)
             ts += extra
             return " * ".join(ts) if ts else "1.0"
         dnu = int(sum(nu for _, nu in r.products)
                   - sum(nu for _, nu in r.reactants))
+        if self.kinv_f:
+            num = side(r.products, ["rtp"] * dnu if dnu > 0 else [])
+            den = side(r.reactants, ["prt"] * (-dnu) if dnu < 0 else [], "F")
+            return f"({num}) * ({den})"
         num_extra = ["rtp"] * dnu if dnu > 0 else []
         den_extra = ["rtp"] * (-dnu) if dnu < 0 else []
         return (f"({side(r.products, num_extra)}) * "
@@ -1223,6 +1237,9 @@ def _emit_wdot_jvp_lazy(w: _Writer, L: _Lowering):
         w("  dCtot += " + dcg.format(k="k") + ";")
         w("}")
     w("double C[NS], dC[NS], E[NS], hrt[NS], w[NS], dw[NS];")
+    L.kinv_f = L.species_kc and L.tuning.get("kinv_f", False)
+    if L.kinv_f:
+        w("double F[NS];")
     order = reaction_order(L, L.tuning.get("order_tries", 8))
     uses = [species_uses(L, r) for r in m.reactions]
     last_w = {}
@@ -1241,6 +1258,8 @@ def _emit_wdot_jvp_lazy(w: _Writer, L: _Lowering):
             w(f"dC[{k}] = " + dcg.format(k=k) + ";")
         for k in sorted(e_u - open_e):
             L.emit_gibbs_one(w, k)
+            if L.kinv_f:
+                w(f"F[{k}] = pyro::recip(E[{k}]);")
             w(f"hrt[{k}] = {L.h_expr(m.species[k])};")
         for k in sorted(w_u - open_w):
             w(f"w[{k}] = 0.0; dw[{k}] = 0.0;")
@@ -1290,6 +1309,7 @@ def _emit_wdot_jvp_lazy(w: _Writer, L: _Lowering):
     w.ind -= 1
     w("}")
     w("")
+    L.kinv_f = False
 
 
 def _emit_wdot_vjp(w: _Writer, L: _Lowering):
@@ -1405,6 +1425,9 @@ def _emit_wdot_vjp_lazy(w: _Writer, L: _Lowering):
         w("#pragma unroll")
         w("for (int k = 0; k < NS; ++k) Ctot += kInvW[k] * rho * Yg[k * ldY + i];")
     w("double y[NS], C[NS], E[NS], hrt[NS], wb[NS], Cb[NS];")
+    L.kinv_f = L.species_kc and L.tuning.get("kinv_f", False)
+    if L.kinv_f:
+        w("double F[NS];")
     w("double tb = 0.0, mbd = 0.0, gr = 0.0;")
     order = reaction_order(L, L.tuning.get("order_tries", 8))
     uses = [species_uses(L, r) for r in m.reactions]
@@ -1428,6 +1451,8 @@ def _emit_wdot_vjp_lazy(w: _Writer, L: _Lowering):
             w(f"wb[{k}] = Bg[{k} * ldB + i];")
         for k in sorted(e_u - open_e):
             L.emit_gibbs_one(w, k)
+            if L.kinv_f:
+                w(f"F[{k}] = pyro::recip(E[{k}]);")
             w(f"hrt[{k}] = {L.h_expr(m.species[k])};")
         open_c |= c_u
         open_e |= e_u
@@ -1481,7 +1506,7 @@ def _emit_wdot_vjp_lazy(w: _Writer, L: _Lowering):
         w("gr = fma(mbd, Ctot / rho, gr);")
         w("if (g_Y) {")
         w.ind += 1
-        w("double* __restrict__ gy = g_Y;")
+        w("pyro::gmptr gy = (pyro::gmptr)g_Y;")
         w('asm volatile("" : "+s"(gy));')
         for k in range(L.ns):
             if k in closed:
@@ -1500,6 +1525,7 @@ def _emit_wdot_vjp_lazy(w: _Writer, L: _Lowering):
     w.ind -= 1
     w("}")
     w("")
+    L.kinv_f = False
 
 
 def cache_slots(L: _Lowering):

@@ -57,6 +57,8 @@ typedef const __attribute__((address_space(4))) double* cptr;
 // Pointer into global memory (keeps loads through a pointer that passed an
 // asm statement on the global path rather than the flat one).
 typedef const __attribute__((address_space(1))) double* gptr;
+// Writable pointer into global memory, for the same reason.
+typedef __attribute__((address_space(1))) double* gmptr;
 // Pointer into LDS (loads through it stay ds_read rather than flat).
 typedef const __attribute__((address_space(3))) double* lptr;
 
