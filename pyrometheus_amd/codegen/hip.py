@@ -635,47 +635,40 @@ def _emit_thermo(w: _Writer, L: _Lowering):
                       L.dh_expr)
     L.emit_species_fn(w, "species_ds_r", "const double T, const double invT",
                       L.ds_expr)
-    # mixture sums sum_k yw_k (h_k/RT - shift) and sum_k yw_k (cp_k/R -
-    # shift), species by species: both NASA ranges evaluated, the result
-    # selected (no per-coefficient selects, no per-species arrays)
-    # The coefficients come from a constant table through a pointer passed
-    # through an empty asm before each species, so they are read where they
-    # are used (scalar loads) rather than hoisted out of the caller's
-    # Newton loop into registers.
+    # mixture NASA-7 coefficients of a cell, for the temperature Newton:
+    # sum_k yw_k h_k/RT and sum_k yw_k cp_k/R are polynomials in T whose
+    # coefficients M_j = sum_k yw_k a_kj do not change between iterations.
+    # Species are grouped by T_mid (the range flag is the same for all the
+    # species of a group); single-range species join group 0 in both
+    # ranges.  The pre-pass reads each species' coefficients from a constant
+    # table through a pointer passed through an empty asm (scalar loads
+    # where they are used); an iteration then costs one polynomial per group.
+    tm = sorted({sp.t_mid for sp in L.m.species if sp.high != sp.low}) or [None]
+    gi = {t: j for j, t in enumerate(tm)}
+    ng = len(tm)
     table_at = len(w.lines)
-    w("__device__ __forceinline__ void mixture_h_cp_sums(const double T, "
-      "const double invT, const double (&yw)[NS], const double shift, "
-      "double& sh, double& sc) {")
+    w(f"constexpr int kMixGroups = {ng};")
+    w("__device__ __forceinline__ void mixture_nasa_coeffs("
+      "const double (&yw)[NS], double (&M)[kMixGroups][2][6], double& syw) {")
     w.ind += 1
     L.tab = []
     w("pyro::cptr K = (pyro::cptr)kTabMixHCp;")
-    L._flags(w)
-    w("sh = 0.0;")
-    w("sc = 0.0;")
+    w("#pragma unroll")
+    w("for (int g = 0; g < kMixGroups; ++g)")
+    w("#pragma unroll")
+    w("  for (int r = 0; r < 2; ++r)")
+    w("#pragma unroll")
+    w("    for (int j = 0; j < 6; ++j) M[g][r][j] = 0.0;")
+    w("syw = 0.0;")
     for k, sp in enumerate(L.m.species):
-        def hpoly(c):
-            cs = [L.c(c[0])] + [L.c(c[j] / d) for j, d in ((1, 2), (2, 3),
-                                                            (3, 4), (4, 5))]
-            return f"fma({L.c(c[5])}, iTk, {L._horner(cs, 'Tk')})"
-
-        def cpoly(c):
-            return L._horner([L.c(c[j]) for j in range(5)], "Tk")
-        # copies of T and of the table pointer passed through an empty asm
-        # after the previous species' sums: the species are evaluated one
-        # after the other instead of all at once
+        g = 0 if sp.high == sp.low else gi[sp.t_mid]
         w(f"{{  // {sp.name}")
-        w("  double Tk = T, iTk = invT;")
-        w('  asm volatile("" : "+v"(Tk), "+v"(iTk), "+s"(K));')
-        if sp.high == sp.low:
-            w(f"  sh = fma(yw[{k}], {hpoly(sp.high)} - shift, sh);")
-            w(f"  sc = fma(yw[{k}], {cpoly(sp.high)} - shift, sc);")
-        else:
-            f = L.tmid_flag[sp.t_mid]
-            w(f"  const double hh = {hpoly(sp.high)}, hl = {hpoly(sp.low)};")
-            w(f"  const double ch = {cpoly(sp.high)}, cl = {cpoly(sp.low)};")
-            w(f"  sh = fma(yw[{k}], ({f} ? hh : hl) - shift, sh);")
-            w(f"  sc = fma(yw[{k}], ({f} ? ch : cl) - shift, sc);")
-        w('  asm volatile("" : "+v"(sh), "+v"(sc));')
+        w('  asm volatile("" : "+s"(K));')
+        w(f"  const double y = yw[{k}];")
+        w("  syw += y;")
+        for r, c in ((0, sp.high), (1, sp.low)):
+            for j in range(6):
+                w(f"  M[{g}][{r}][{j}] = fma(y, {L.c(c[j])}, M[{g}][{r}][{j}]);")
         w("}")
     w.ind -= 1
     w("}")
@@ -683,6 +676,32 @@ def _emit_thermo(w: _Writer, L: _Lowering):
     w.lines.insert(table_at, "__constant__ double kTabMixHCp[] = {"
                    + ", ".join(lit(x) for x in tab) + "};")
     L.tab = None
+    # sum_k yw_k (h_k/RT - shift) and sum_k yw_k (cp_k/R - shift) at T from
+    # the mixture coefficients: h/RT = a0 + a1 T/2 + a2 T^2/3 + a3 T^3/4 +
+    # a4 T^4/5 + a5/T, cp/R = a0 + a1 T + a2 T^2 + a3 T^3 + a4 T^4
+    w("__device__ __forceinline__ void mixture_h_cp_eval(const double T, "
+      "const double invT, const double (&M)[kMixGroups][2][6], "
+      "const double syw, const double shift, double& sh, double& sc) {")
+    w.ind += 1
+    L._flags(w)
+    w("double h = 0.0, c = 0.0;")
+    for g, t in enumerate(tm):
+        sel = L.tmid_flag[t] if t is not None else "true"
+        w("{")
+        w.ind += 1
+        w(f"const bool hi = {sel};")
+        w("double m[6];")
+        w("#pragma unroll")
+        w(f"for (int j = 0; j < 6; ++j) m[j] = hi ? M[{g}][0][j] : M[{g}][1][j];")
+        w("h += fma(m[5], invT, fma(T, fma(T, fma(T, fma(T, 0.2 * m[4], "
+          "0.25 * m[3]), (1.0 / 3.0) * m[2]), 0.5 * m[1]), m[0]));")
+        w("c += fma(T, fma(T, fma(T, fma(T, m[4], m[3]), m[2]), m[1]), m[0]);")
+        w.ind -= 1
+        w("}")
+    w("sh = fma(-shift, syw, h);")
+    w("sc = fma(-shift, syw, c);")
+    w.ind -= 1
+    w("}")
     w("__device__ __forceinline__ void species_g_rt(const double T, "
       "const double lnT, const double invT, double (&o)[NS]) {")
     w.ind += 1

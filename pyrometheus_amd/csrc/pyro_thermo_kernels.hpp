@@ -338,15 +338,20 @@ k_temperature(const double* __restrict__ target, int64_t target_stride,
   const double e = target[i * target_stride];
   double t = t_guess[i * guess_stride];
   const double shift = do_energy != 0 ? 1.0 : 0.0;
-  double yw[mech::NS];
+  // the mixture's NASA-7 coefficients, once per cell
+  double M[mech::kMixGroups][2][6], syw;
+  {
+    double yw[mech::NS];
 #pragma unroll
-  for (int k = 0; k < mech::NS; ++k) yw[k] = Y[k * ldY + i] * mech::kInvW[k];
+    for (int k = 0; k < mech::NS; ++k) yw[k] = Y[k * ldY + i] * mech::kInvW[k];
+    mech::mixture_nasa_coeffs(yw, M, syw);
+  }
   int it = 0;
   bool ok = false;
   for (; it < maxit; ++it) {
-    // he = e or h (mass), pv = cv or cp (mass), one pass over the species
+    // he = e or h (mass), pv = cv or cp (mass), one polynomial per group
     double sh, sc;
-    mech::mixture_h_cp_sums(t, 1.0 / t, yw, shift, sh, sc);
+    mech::mixture_h_cp_eval(t, 1.0 / t, M, syw, shift, sh, sc);
     const double he = kGasConstant * t * sh;
     const double pv = kGasConstant * sc;
     const double dt = (e - he) / pv;  // = -rhs/deriv with deriv = -pv
