@@ -1754,6 +1754,10 @@ def _emit_reaction_quantities(w: _Writer, L: _Lowering):
     m = L.m
     w("// per-reaction k_f, -ln K_c, r_f, r_r, r_net to memory "
       "(codegen/python.py:360-515)")
+    # MASK selects the outputs at compile time (1 k_f, 2 -ln K_c, 4 r_f,
+    # 8 r_r, 16 r_net), so a call for k_f alone does not evaluate the Gibbs
+    # functions or the reverse rates
+    w("template <int MASK>")
     w("__device__ __forceinline__ void reaction_quantities(const double T, "
       "const double (&C)[NS], const int64_t i, const int64_t ld, "
       "double* __restrict__ okf, double* __restrict__ onlk, "
@@ -1771,18 +1775,18 @@ def _emit_reaction_quantities(w: _Writer, L: _Lowering):
             w(f"const double nlk = {L.nlk_expr(r)};")
             w(f"const double pf = {rf};")
             w(f"const double pb = {rr};")
-            w(f"if (okf) okf[{j} * ld + i] = kf;")
-            w(f"if (onlk) onlk[{j} * ld + i] = nlk;")
-            w(f"if (orf) orf[{j} * ld + i] = kf * pf;")
-            w(f"if (orr) orr[{j} * ld + i] = kf * exp(nlk) * pb;")
-            w(f"if (onet) onet[{j} * ld + i] = kf * (pf - exp(nlk) * pb);")
+            w(f"if ((MASK & 1) && okf) okf[{j} * ld + i] = kf;")
+            w(f"if ((MASK & 2) && onlk) onlk[{j} * ld + i] = nlk;")
+            w(f"if ((MASK & 4) && orf) orf[{j} * ld + i] = kf * pf;")
+            w(f"if ((MASK & 8) && orr) orr[{j} * ld + i] = kf * exp(nlk) * pb;")
+            w(f"if ((MASK & 16) && onet) onet[{j} * ld + i] = kf * (pf - exp(nlk) * pb);")
         else:
             w(f"const double pf = {rf};")
-            w(f"if (okf) okf[{j} * ld + i] = kf;")
-            w(f"if (onlk) onlk[{j} * ld + i] = -0.17364695002734 * T;")
-            w(f"if (orf) orf[{j} * ld + i] = kf * pf;")
-            w(f"if (orr) orr[{j} * ld + i] = 0.0;")
-            w(f"if (onet) onet[{j} * ld + i] = kf * pf;")
+            w(f"if ((MASK & 1) && okf) okf[{j} * ld + i] = kf;")
+            w(f"if ((MASK & 2) && onlk) onlk[{j} * ld + i] = -0.17364695002734 * T;")
+            w(f"if ((MASK & 4) && orf) orf[{j} * ld + i] = kf * pf;")
+            w(f"if ((MASK & 8) && orr) orr[{j} * ld + i] = 0.0;")
+            w(f"if ((MASK & 16) && onet) onet[{j} * ld + i] = kf * pf;")
         w.ind -= 1
         w("}")
     w.ind -= 1

@@ -159,6 +159,9 @@ __device__ __forceinline__ void load_conc(const double* __restrict__ rho,
   for (int k = 0; k < mech::NS; ++k) C[k] = mech::kInvW[k] * r * Y[k * ldY + i];
 }
 
+// MASK: the outputs computed (1 kf, 2 nlk, 4 rf, 8 rr, 16 rnet), each
+// also skipped when its pointer is null.
+template <int MASK>
 __global__ void __launch_bounds__(kBlock)
 k_reaction_quantities(const double* __restrict__ T,
                       const double* __restrict__ C, int64_t ldC,
@@ -170,7 +173,7 @@ k_reaction_quantities(const double* __restrict__ T,
   double c[mech::NS];
 #pragma unroll
   for (int k = 0; k < mech::NS; ++k) c[k] = C[k * ldC + i];
-  mech::reaction_quantities(T[i], c, i, ldR, kf, nlk, rf, rr, rnet);
+  mech::reaction_quantities<MASK>(T[i], c, i, ldR, kf, nlk, rf, rr, rnet);
 }
 
 // kWdotWaves > 0 asks the compiler for that many waves per SIMD (it then
@@ -465,8 +468,21 @@ PYRO_API int pyro_reaction_quantities(const double* T, const double* C,
   using namespace pyro_kern;
   if (n == 0) return pyro::kOk;
   if (n < 0 || ldC < n || ldR < n || !T || !C) return pyro::kInvalidArgument;
-  k_reaction_quantities<<<pyro::grid_for(n, kBlock), kBlock, 0, stream>>>(
-      T, C, ldC, kf, nlk, rf, rr, rnet, ldR, n);
+  const int mask = (kf ? 1 : 0) | (nlk ? 2 : 0) | (rf ? 4 : 0) |
+                   (rr ? 8 : 0) | (rnet ? 16 : 0);
+  const unsigned grid = pyro::grid_for(n, kBlock);
+  if (mask == 1)
+    k_reaction_quantities<1><<<grid, kBlock, 0, stream>>>(
+        T, C, ldC, kf, nlk, rf, rr, rnet, ldR, n);
+  else if (mask == 2)
+    k_reaction_quantities<2><<<grid, kBlock, 0, stream>>>(
+        T, C, ldC, kf, nlk, rf, rr, rnet, ldR, n);
+  else if (mask == 4)
+    k_reaction_quantities<4><<<grid, kBlock, 0, stream>>>(
+        T, C, ldC, kf, nlk, rf, rr, rnet, ldR, n);
+  else
+    k_reaction_quantities<31><<<grid, kBlock, 0, stream>>>(
+        T, C, ldC, kf, nlk, rf, rr, rnet, ldR, n);
   return pyro::launch_status();
 }
 
