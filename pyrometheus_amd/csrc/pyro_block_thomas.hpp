@@ -39,34 +39,67 @@ __device__ __forceinline__ double bcast(double v, int lane) {
                               static_cast<unsigned int>(lo));
 }
 
-// Max over lanes 0..15 of a double, by DPP rotations within the first
-// 16-lane row (lanes 16..63 rotate among themselves and are ignored).
-__device__ __forceinline__ double row16_max(double v) {
-#define PYRO_ROR_MAX(N)                                                      \
+// Pivot key of a row for column k: the top bits of |a| (exponent and 15
+// mantissa bits, inf capped to the largest finite value) above the lane
+// index inverted, so that an integer max picks the largest |a| and, among
+// rows equal to 2^-15 relative, the lowest lane -- as good a pivot for
+// partial pivoting.  Rows that are not candidates, and NaN rows, get 0: a
+// maximum below 64 means no usable row is left, and the caller takes the
+// lowest unused row.
+__device__ __forceinline__ unsigned pivot_key(double v, bool cand, int lane) {
+  const double a = fabs(v);
+  if (!cand || !(a == a)) return 0u;
+  unsigned hi = static_cast<unsigned>(__double_as_longlong(a) >> 32);
+  hi = hi < 0x7FEFFFFFu ? hi : 0x7FEFFFFFu;
+  return (((hi >> 5) + 1u) << 6) | static_cast<unsigned>(63 - lane);
+}
+
+// Max over each 16-lane row of a 32-bit key, in every lane of the row (DPP
+// rotations within the row, one v_max_u32 each).
+__device__ __forceinline__ unsigned row16_max_u32(unsigned v) {
+#define PYRO_ROR_MAXU(N)                                                     \
   {                                                                          \
-    const long long b = __double_as_longlong(v);                             \
-    const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b), 0x120 + N,  \
-                                            0xF, 0xF, false);                \
-    const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32),       \
-                                            0x120 + N, 0xF, 0xF, false);     \
-    v = fmax(v, __longlong_as_double((static_cast<long long>(hi) << 32) |    \
-                                     static_cast<unsigned int>(lo)));        \
+    const unsigned o = static_cast<unsigned>(__builtin_amdgcn_mov_dpp(       \
+        static_cast<int>(v), 0x120 + N, 0xF, 0xF, false));                   \
+    v = v > o ? v : o;                                                       \
   }
-  PYRO_ROR_MAX(8)
-  PYRO_ROR_MAX(4)
-  PYRO_ROR_MAX(2)
-  PYRO_ROR_MAX(1)
-#undef PYRO_ROR_MAX
+  PYRO_ROR_MAXU(8)
+  PYRO_ROR_MAXU(4)
+  PYRO_ROR_MAXU(2)
+  PYRO_ROR_MAXU(1)
+#undef PYRO_ROR_MAXU
   return v;
+}
+
+// Lane of the wave-wide maximum key (wave-uniform), or the lowest unused
+// row when no usable row is left.
+template <int NV>
+__device__ __forceinline__ int pivot_lane(unsigned key,
+                                          unsigned long long used,
+                                          unsigned long long all) {
+  constexpr int kRows = (NV + 15) / 16;
+  const unsigned rm = row16_max_u32(key);
+  unsigned best = static_cast<unsigned>(
+      __builtin_amdgcn_readlane(static_cast<int>(rm), 0));
+#pragma unroll
+  for (int q = 1; q < kRows; ++q) {
+    const unsigned b = static_cast<unsigned>(
+        __builtin_amdgcn_readlane(static_cast<int>(rm), 16 * q));
+    best = best > b ? best : b;
+  }
+  const int p = best >= 64u ? 63 - static_cast<int>(best & 63u)
+                            : __builtin_ctzll(~used & all);
+  return __builtin_amdgcn_readfirstlane(p);
 }
 
 // Gauss-Jordan on the augmented rows a[0..W) held one per lane (lanes < NV).
 // On return the lane's row is logical row `my_k` of [I | X]; `perm[k]` is
-// the lane holding logical row k (wave-uniform).  Each lane first scales its row so that its largest |C'| entry is 1 (the
-// solution does not change; pivoting becomes scaled partial pivoting).  The
-// pivot is found by a DPP max-reduction of |a[k]| within 16-lane rows (and
-// a max over the rows' results) with the lane index packed into the low
-// mantissa bits, the reciprocal of the pivot by v_rcp_f64 and two Newton
+// the lane holding logical row k (wave-uniform).  Each lane first scales
+// its row so that its largest |C'| entry is 1 (the solution does not
+// change; pivoting becomes scaled partial pivoting).  The pivot is the
+// lane of the largest pivot_key(a[k]) (a DPP max-reduction within 16-lane
+// rows and a max over the rows' results), the reciprocal of the pivot by
+// v_rcp_f64 and two Newton
 // steps, and every lane -- the pivot lane included -- updates its row with
 // one FMA per column: the pivot lane uses the multiplier 1 - 1/piv, which
 // turns a - (1 - 1/piv) a into a/piv with a relative error of |piv - 1| eps,
@@ -76,7 +109,6 @@ __device__ __forceinline__ void gauss_jordan_dpp(double (&a)[W], int lane,
                                                  int& my_k, int (&perm)[NV],
                                                  bool& singular) {
   static_assert(NV <= 64, "one wavefront holds the rows");
-  constexpr int kRows = (NV + 15) / 16;
   const bool active = lane < NV;
   {
     double mx = 0.0;
@@ -92,23 +124,8 @@ __device__ __forceinline__ void gauss_jordan_dpp(double (&a)[W], int lane,
   my_k = -1;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    double key = -1.0;
-    if (active && !((used >> lane) & 1ull)) {
-      const double v = fabs(a[k]);
-      key = (v == v) ? v : -0.5;  // a NaN row is taken only as a last resort
-    }
-    long long kb = __double_as_longlong(key);
-    if (key >= 0.0) kb = (kb & ~0x3Fll) | (63 - lane);
-    const double rm = row16_max(__longlong_as_double(kb));
-    double best = bcast(rm, 0);
-#pragma unroll
-    for (int q = 1; q < kRows; ++q) best = fmax(best, bcast(rm, 16 * q));
-    int p;
-    if (best >= 0.0)
-      p = 63 - static_cast<int>(__double_as_longlong(best) & 0x3F);
-    else
-      p = __builtin_ctzll(~used & all);
-    p = __builtin_amdgcn_readfirstlane(p);
+    const int p = pivot_lane<NV>(
+        pivot_key(a[k], active && !((used >> lane) & 1ull), lane), used, all);
     used |= 1ull << p;
     perm[k] = p;
     const double piv = bcast(a[k], p);

@@ -58,20 +58,11 @@ __device__ __forceinline__ void gauss_jordan_grp(double (&a)[W], int lane,
   my_k = -1;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    double key = -1.0;
-    if (active && !((used >> gl) & 1u)) {
-      const double v = fabs(a[k]);
-      key = (v == v) ? v : -0.5;  // a NaN row is taken only as a last resort
-    }
-    long long kb = __double_as_longlong(key);
-    if (key >= 0.0) kb = (kb & ~0x3Fll) | (63 - lane);
-    // every lane of the 16-lane row holds the row's maximum
-    const double best = pyro_bt::row16_max(__longlong_as_double(kb));
-    int p;
-    if (best >= 0.0)
-      p = 63 - static_cast<int>(__double_as_longlong(best) & 0x3F) - base;
-    else
-      p = __builtin_ctz(~used & all);
+    // every lane of the 16-lane row holds the row's maximum key
+    const unsigned best = pyro_bt::row16_max_u32(
+        pyro_bt::pivot_key(a[k], active && !((used >> gl) & 1u), lane));
+    const int p = best >= 64u ? 63 - static_cast<int>(best & 63u) - base
+                              : __builtin_ctz(~used & all);
     used |= 1u << p;
     const int src = base + p;
     const double piv = __shfl(a[k], src);
@@ -410,7 +401,6 @@ __device__ __forceinline__ void gj_waves(double (&a0)[T], double (&ab)[NB][T],
                                          double& x, int lane, int wv,
                                          int& my_k, GjShared& sh) {
   static_assert(NV <= 64, "one wavefront holds the rows");
-  constexpr int kRows = (NV + 15) / 16;
   const bool active = lane < NV;
   {
     double mx = 0.0;
@@ -437,23 +427,9 @@ __device__ __forceinline__ void gj_waves(double (&a0)[T], double (&ab)[NB][T],
   for (int k = 0; k < NV; ++k) {
     const int tk = k / kWaves;
     if (wv == k % kWaves) {
-      double key = -1.0;
-      if (active && !((used >> lane) & 1ull)) {
-        const double v = fabs(a0[tk]);
-        key = (v == v) ? v : -0.5;  // a NaN row only as a last resort
-      }
-      long long kb = __double_as_longlong(key);
-      if (key >= 0.0) kb = (kb & ~0x3Fll) | (63 - lane);
-      const double rm = pyro_bt::row16_max(__longlong_as_double(kb));
-      double best = bcast(rm, 0);
-#pragma unroll
-      for (int q = 1; q < kRows; ++q) best = fmax(best, bcast(rm, 16 * q));
-      int p;
-      if (best >= 0.0)
-        p = 63 - static_cast<int>(__double_as_longlong(best) & 0x3F);
-      else
-        p = __builtin_ctzll(~used & all);
-      p = __builtin_amdgcn_readfirstlane(p);
+      const int p = pyro_bt::pivot_lane<NV>(
+          pyro_bt::pivot_key(a0[tk], active && !((used >> lane) & 1ull), lane),
+          used, all);
       const double piv = bcast(a0[tk], p);
       singular = singular || !(fabs(piv) > 0.0);
       const double inv = pyro::recip(piv);
