@@ -96,6 +96,41 @@ def test_gpu_batched_ragged(nv, diag, device):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("nv", [3, 10, 54])
+def test_gpu_singular_block_reported(nv, device):
+    """A system whose first central block is zero meets a zero pivot: with
+    check_singular the batched solve raises; the same batch without the
+    zero block solves to the oracle."""
+    import torch
+    from pyrometheus_amd.flamelets.linear_solver import block_thomas_batched
+    rng = np.random.default_rng(7 + nv)
+    sizes = [4, 6]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    P = offs[-1]
+    Lb, Ub = np.zeros(P), np.zeros(P)
+    Cb, Rb = np.zeros((P, nv, nv)), np.zeros((P, nv))
+    refs = []
+    for o, n in zip(offs, sizes):
+        L, C, U, r = _system(rng, n, nv)
+        lv, uv = 0.1 * rng.normal(size=n - 1), 0.1 * rng.normal(size=n - 1)
+        Lb[o:o + n - 1], Ub[o:o + n - 1] = lv, uv
+        Cb[o:o + n], Rb[o:o + n] = C, r
+        refs.append(block_thomas(lv[:, None, None] * np.eye(nv), C,
+                                 uv[:, None, None] * np.eye(nv), r))
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=device)
+    offs_t = torch.tensor(offs, device=device)
+    x = block_thomas_batched(t(Lb), t(Cb), t(Ub), t(Rb), offs_t, diag=True,
+                             check_singular=True).cpu().numpy()
+    for ref, o, n in zip(refs, offs, sizes):
+        assert np.max(np.abs(x[o:o + n] - ref)) <= 1e-12 * np.max(np.abs(ref))
+    Cs = Cb.copy()
+    Cs[offs[1]] = 0.0
+    with pytest.raises(RuntimeError, match="singular"):
+        block_thomas_batched(t(Lb), t(Cs), t(Ub), t(Rb), offs_t, diag=True,
+                             check_singular=True)
+
+
+@pytest.mark.gpu
 def test_gpu_drop_in_signature(device):
     import torch
     from pyrometheus_amd.flamelets.linear_solver import block_thomas as bt
