@@ -126,11 +126,13 @@ class _Writer:
 #:           without a reciprocal per reaction (off: the extra live values
 #:           cost more than the reciprocals, JVP 9.9 -> 10.3 ms and VJP
 #:           8.7 -> 9.0 ms on 2^22 GRI-3.0 cells);
-#:  jac_groups -- waves of a k_source_jacobian_rows block that split the
-#:           rows of the same cells (1; 2 and 4, with one or two waves per
-#:           SIMD, measured 14-25 % slower than 1: the row accumulators,
-#:           the concentrations and the hoisted cache loads outgrow 256
-#:           registers);
+#:  jac_groups -- groups of rows (balanced by work) in the source-Jacobian
+#:           row pass: a large chunk runs every group in one thread (one
+#:           wave per SIMD), a small one (a flamelet batch) gives each group
+#:           its own wave for the same cells, so that more waves share the
+#:           work (whole-chunk runs with split groups measured 14-25 %
+#:           slower: the row accumulators, the concentrations and the
+#:           hoisted cache loads outgrow 256 registers);
 #:  jac_waves -- waves per SIMD the compiler must budget registers for in
 #:           k_source_jacobian_rows (0: its own choice);
 #:  jac_fence -- an empty asm with a memory clobber every that many
@@ -140,7 +142,7 @@ class _Writer:
 DEFAULT_TUNING = {"pin": True, "pin_every": 3, "kc": "species", "ctable": True,
                   "lazy": True, "wdot_waves": 2, "deriv_waves": 0,
                   "lazy_fence": 1, "order_tries": 8, "prefetch": 4,
-                  "zform": True}
+                  "zform": True, "jac_groups": 4}
 
 
 def element_shift(mech: Mechanism):
@@ -1612,14 +1614,16 @@ def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
     m = L.m
     slots, _ = cache_slots(L)
     ns = L.ns
-    w("// rows of the source Jacobian from the reaction cache")
+    w("// rows of the source Jacobian from the reaction cache: the row groups")
+    w("// G0..G1-1 (compile-time), one after the other")
+    w("template <int G0, int G1>")
     w("__device__ __forceinline__ void source_jacobian_rows("
       "double (&C)[NS], const double rho, const double T, "
       "const double cp, const double mmw, const double vd, "
       "const double* __restrict__ work, const unsigned il, "
       "pyro::lptr dTl, const int hs, "
       "double* __restrict__ o, const int64_t s_rc, "
-      "double* __restrict__ S, const int64_t ldS, const int rg) {")
+      "double* __restrict__ S, const int64_t ldS) {")
     w.ind += 1
     w("pyro::cptr IW = (pyro::cptr)kInvWMem;")
     # the cache as a global-memory pointer: loads from a uniform base plus
@@ -1647,27 +1651,21 @@ def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
         g = min(range(ng), key=lambda g: load[g])
         group_of[k] = g
         load[g] += len(inv[k]) * 8 + ns
-    w("if (rg == 0) {")
-    w.ind += 1
-    w("{  // row 0: d(vd/rho)")
-    w.ind += 1
-    w("const double v0 = -vd * irho * irho;")
-    w("o[0] = v0 * dr0;")
-    w("#pragma unroll")
-    w("for (int q = 0; q < NS; ++q) {")
-    w("  const double dT = dTl[q * hs];")
-    w("  o[(1 + q) * s_rc] = v0 * fma(-rT, dT, rm * kInvW[q]);")
-    w("}")
-    w("if (S) S[0] = vd * irho;")
-    w.ind -= 1
-    w("}")
-    cur = 0
-    for k in sorted(rows, key=lambda k: (group_of[k], k)):
-        if group_of[k] != cur:
-            cur = group_of[k]
-            w.ind -= 1
-            w(f"}} else if (rg == {cur}) {{")
-            w.ind += 1
+    def emit_row0():
+        w("{  // row 0: d(vd/rho)")
+        w.ind += 1
+        w("const double v0 = -vd * irho * irho;")
+        w("o[0] = v0 * dr0;")
+        w("#pragma unroll")
+        w("for (int q = 0; q < NS; ++q) {")
+        w("  const double dT = dTl[q * hs];")
+        w("  o[(1 + q) * s_rc] = v0 * fma(-rT, dT, rm * kInvW[q]);")
+        w("}")
+        w("if (S) S[0] = vd * irho;")
+        w.ind -= 1
+        w("}")
+
+    def emit_row(k):
         w(f"{{  // row {k + 1}: {m.species[k].name}")
         w.ind += 1
         # fresh copies of C and of the per-cell scalars for this row:
@@ -1743,7 +1741,44 @@ def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
         w('asm volatile("" ::: "memory");')
         w.ind -= 1
         w("}")
+
+    # one thread per cell: every row, in natural order
+    w(f"if constexpr (G0 == 0 && G1 == {ng}) {{")
+    w.ind += 1
+    emit_row0()
+    for k in rows:
+        emit_row(k)
     w.ind -= 1
+    w("} else {")
+    w.ind += 1
+    # one wave per group of rows (the split form)
+    for g in range(ng):
+        w(f"if constexpr (G0 <= {g} && {g} < G1) {{")
+        w.ind += 1
+        if g == 0:
+            emit_row0()
+        for k in sorted(r for r in rows if group_of[r] == g):
+            emit_row(k)
+        w.ind -= 1
+        w("}")
+    w.ind -= 1
+    w("}")
+    w.ind -= 1
+    w("}")
+    # one group per wave: the group of wave rg (wave-uniform)
+    w("__device__ __forceinline__ void source_jacobian_rows_group(int rg, "
+      "double (&C)[NS], const double rho, const double T, "
+      "const double cp, const double mmw, const double vd, "
+      "const double* __restrict__ work, const unsigned il, "
+      "pyro::lptr dTl, const int hs, "
+      "double* __restrict__ o, const int64_t s_rc, "
+      "double* __restrict__ S, const int64_t ldS) {")
+    w.ind += 1
+    w("switch (rg) {")
+    for g in range(ng):
+        w(f"  case {g}: source_jacobian_rows<{g}, {g + 1}>(C, rho, T, cp, mmw, "
+          "vd, work, il, dTl, hs, o, s_rc, S, ldS); break;")
+    w("  default: break;")
     w("}")
     w.ind -= 1
     w("}")

@@ -269,12 +269,13 @@ k_reaction_cache(const double* __restrict__ T, const double* __restrict__ p,
   mech::reaction_cache(t, C, work, static_cast<unsigned>(i));
 }
 
-// Cells per block of k_source_jacobian_rows: each of the block's
-// mech::kJacGroups waves builds one group of rows (balanced by work) for
-// the same kJacCells cells, so the dT table in LDS is shared by the groups.
-constexpr int kJacCells = kBlock / mech::kJacGroups;
-static_assert(kJacCells % 64 == 0, "a row group is whole waves");
-
+// Row pass of the source Jacobian.  G = 1: one thread per cell builds
+// every row (kBlock cells per block); G = mech::kJacGroups: the G waves of
+// a block build one group of rows each (balanced by work) for the same
+// kBlock / G cells, sharing the dT table in LDS -- G times the waves for a
+// chunk of few cells (a flamelet batch).  A row's arithmetic is the same
+// in both forms.
+template <int G>
 __global__ void __launch_bounds__(kBlock, mech::kJacWaves > 0 ? mech::kJacWaves : 1)
 k_source_jacobian_rows(const double* __restrict__ T,
                        const double* __restrict__ p, int64_t p_stride,
@@ -283,11 +284,13 @@ k_source_jacobian_rows(const double* __restrict__ T,
                        const double* __restrict__ work,
                        double* __restrict__ out, int64_t s_rc, int64_t s_i,
                        double* __restrict__ S, int64_t ldS, int64_t n) {
+  constexpr int kCells = kBlock / G;
+  static_assert(kCells % 64 == 0, "a row group is whole waves");
   // dT of column 1+m for every cell of the block, read by every row
-  __shared__ double dTl[mech::NS * kJacCells];
-  const int cl = threadIdx.x % kJacCells;
-  const int rg = __builtin_amdgcn_readfirstlane(threadIdx.x / kJacCells);
-  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kJacCells + cl;
+  __shared__ double dTl[mech::NS * kCells];
+  const int cl = threadIdx.x % kCells;
+  const int rg = __builtin_amdgcn_readfirstlane(threadIdx.x / kCells);
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kCells + cl;
   // lanes past the end work on the last cell (no early return before the
   // barrier) and store nothing
   const int64_t i = i0 < n ? i0 : n - 1;
@@ -308,19 +311,24 @@ k_source_jacobian_rows(const double* __restrict__ T,
     double hrt[mech::NS];
     mech::species_h_rt(t, 1.0 / t, hrt);
     const double kT = -kGasConstant * t / cp;
-    // the groups write the species k = rg (mod groups)
+    // the waves write the species k = rg (mod G)
 #pragma unroll
     for (int k = 0; k < mech::NS; ++k)
-      if (k % mech::kJacGroups == rg)
-        dTl[k * kJacCells + cl] = kT * hrt[k] * mech::kInvW[k];
+      if (k % G == rg)
+        dTl[k * kCells + cl] = kT * hrt[k] * mech::kInvW[k];
   }
   __syncthreads();
   if (i0 >= n) return;
-  mech::source_jacobian_rows(C, rho, t, cp, mmw, vd ? vd[i] : 0.0, work,
-                             static_cast<unsigned>(i),
-                             (pyro::lptr)(dTl + cl), kJacCells,
-                             out + i * s_i, s_rc, S ? S + i : nullptr, ldS,
-                             rg);
+  if constexpr (G == 1)
+    mech::source_jacobian_rows<0, mech::kJacGroups>(
+        C, rho, t, cp, mmw, vd ? vd[i] : 0.0, work, static_cast<unsigned>(i),
+        (pyro::lptr)(dTl + cl), kCells, out + i * s_i, s_rc,
+        S ? S + i : nullptr, ldS);
+  else
+    mech::source_jacobian_rows_group(
+        rg, C, rho, t, cp, mmw, vd ? vd[i] : 0.0, work,
+        static_cast<unsigned>(i), (pyro::lptr)(dTl + cl), kCells,
+        out + i * s_i, s_rc, S ? S + i : nullptr, ldS);
 }
 
 // ---------------------------------------------------------------------------
@@ -559,6 +567,12 @@ PYRO_API int pyro_wdot_vjp(const double* rho, const double* T, const double* Y,
   return pyro::launch_status();
 }
 
+// Chunks of at most this many cells use the split row pass: in the
+// one-thread-per-cell form they would leave at least half of MI355X's
+// 1,024 SIMDs without a wave (a full chunk gives each SIMD one, and there
+// the split form measured slower).
+constexpr int64_t kJacSplitCells = 512 * 64;
+
 PYRO_API int64_t pyro_source_jacobian_work_doubles(void) {
   return static_cast<int64_t>(mech::kCacheSlots) * mech::kCacheLd;
 }
@@ -579,11 +593,20 @@ PYRO_API int pyro_source_jacobian(const double* T, const double* p,
     const int64_t nc = n - c0 < chunk ? n - c0 : chunk;
     k_reaction_cache<<<pyro::grid_for(nc, kBlock), kBlock, 0, stream>>>(
         T + c0, p + c0 * p_stride, p_stride, Y + c0, ldY, work, nc);
-    k_source_jacobian_rows<<<pyro::grid_for(nc, kJacCells), kBlock, 0,
-                             stream>>>(
-        T + c0, p + c0 * p_stride, p_stride, Y + c0, ldY,
-        vd ? vd + c0 : nullptr, work, out + c0 * s_i, s_rc, s_i,
-        S ? S + c0 : nullptr, ldS, nc);
+    // a small chunk splits every cell's rows over the waves of a block
+    constexpr int G = mech::kJacGroups;
+    if (G > 1 && nc <= kJacSplitCells)
+      k_source_jacobian_rows<G><<<pyro::grid_for(nc, kBlock / G), kBlock, 0,
+                                  stream>>>(
+          T + c0, p + c0 * p_stride, p_stride, Y + c0, ldY,
+          vd ? vd + c0 : nullptr, work, out + c0 * s_i, s_rc, s_i,
+          S ? S + c0 : nullptr, ldS, nc);
+    else
+      k_source_jacobian_rows<1><<<pyro::grid_for(nc, kBlock), kBlock, 0,
+                                  stream>>>(
+          T + c0, p + c0 * p_stride, p_stride, Y + c0, ldY,
+          vd ? vd + c0 : nullptr, work, out + c0 * s_i, s_rc, s_i,
+          S ? S + c0 : nullptr, ldS, nc);
     const int st = pyro::launch_status();
     if (st != pyro::kOk) return st;
   }
