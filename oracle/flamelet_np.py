@@ -77,10 +77,17 @@ def laplacian_coefficients(x):
 
 
 def apply_laplacian(abc, f):
+    """Interior rows as ``a (f_{i-1} - f_i) + c (f_{i+1} - f_i)`` (the same
+    operator, ``b = -(a + c)``): the neighbour differences are exact for a
+    smooth f, so the rounding floor scales with the differences rather
+    than with f -- which matters for the enthalpy row, where f ~ 1e5 J/kg
+    and the exact Lap(h) is 0.  The boundary rows are ``b f``."""
     a, b, c = abc
-    out = b * f
-    out[..., 1:] += a[1:] * f[..., :-1]
-    out[..., :-1] += c[:-1] * f[..., 1:]
+    out = np.empty(np.shape(f), dtype=np.result_type(f, float))
+    out[..., 0] = b[0] * f[..., 0]
+    out[..., -1] = b[-1] * f[..., -1]
+    out[..., 1:-1] = (a[1:-1] * (f[..., :-2] - f[..., 1:-1])
+                      + c[1:-1] * (f[..., 2:] - f[..., 1:-1]))
     return out
 
 
@@ -190,10 +197,56 @@ def temperature_from_enthalpy(gas, h, Y, t_guess, tol=1e-10, maxit=100,
     raise RuntimeError("Temperature iteration failed to converge")
 
 
-class FlameletEquations:
-    """Residual and block Jacobian of the steady flamelet equations."""
+def temperature_from_enthalpy_jax(gas, h, Y, t_guess, tol=1e-10,
+                                  maxit=500, pointwise=True):
+    """``PyroJAX.get_temperature_from_enthalpy`` (``flamelets/make_pyro.py:
+    255-299``) as the reference runs it: a ``while_loop`` that computes the
+    step, stops *before* taking it once its norm is ``<= tol``, and carries
+    forward-mode tangents through exactly the iterations it takes.  Under
+    ``vmap`` (the per-point source Jacobian, ``equations.py:124-129``) the
+    norm is per point (``pointwise``); on a whole field (``rhs``) it is the
+    2-norm over the field and every point steps until that is small.
 
-    def __init__(self, mech, x, y_ox, y_fu):
+    Complex inputs carry the tangent in their imaginary part: where the guess
+    already satisfies the test no iteration runs and dT/dphi is exactly 0,
+    as ``jacfwd`` through the reference's loop gives."""
+    T = np.array(t_guess * np.ones_like(np.real(h)),
+                 dtype=np.result_type(h, Y, float))
+    active = np.ones(T.shape, dtype=bool)
+    for _ in range(maxit):
+        f = h - gas.get_mixture_enthalpy_mass(T, Y)
+        step = f / gas.get_mixture_specific_heat_cp_mass(T, Y)
+        if pointwise:
+            active = active & (np.abs(np.real(step)) > tol)
+        elif np.linalg.norm(np.real(np.atleast_1d(step))) <= tol:
+            return T
+        if not active.any():
+            return T
+        T = np.where(active, T + step, T)
+    raise RuntimeError("Temperature iteration failed to converge")
+
+
+class FlameletEquations:
+    """Residual and block Jacobian of the steady flamelet equations.
+
+    ``semantics`` selects the temperature recovery inside them:
+
+    * ``"exact"`` (default): per-point Newton to ``1e-10`` K, and a source
+      Jacobian that is the exact derivative of S (the derivative of T(h, Y)
+      fully included), which is what the device kernels compute;
+    * ``"reference"``: the reference's loops as JAX runs them
+      (:func:`temperature_from_enthalpy_jax`): the field's T by the
+      batch-norm loop and the Jacobian by forward-mode through the per-point
+      loop started from ``t_guess`` -- so where ``t_guess`` is already the
+      converged temperature of a point (the first Newton iteration after a
+      time march, and the first iteration of every Crank-Nicolson step)
+      the Jacobian omits dT/dphi there.  This reproduces the reference's
+      Newton iterates, not only its converged state."""
+
+    def __init__(self, mech, x, y_ox, y_fu, semantics="exact"):
+        if semantics not in ("exact", "reference"):
+            raise ValueError("semantics must be 'exact' or 'reference'")
+        self.semantics = semantics
         self.mech = mech
         self.gas = Thermochemistry(mech)
         self.x = np.asarray(x, dtype=float)
@@ -212,10 +265,15 @@ class FlameletEquations:
         bc[1:, 0], bc[1:, -1] = self.y_ox, self.y_fu
         return bc
 
-    def source_terms(self, phi, vd, t_guess, p, extra=0):
+    def source_terms(self, phi, vd, t_guess, p, extra=0, pointwise=True):
         """S = [vd/rho; W_k wdot_k / rho] (equations.py:313-359)."""
         h, Y = phi[0], phi[1:]
-        T = temperature_from_enthalpy(self.gas, h, Y, t_guess, extra=extra)
+        if self.semantics == "reference":
+            T = temperature_from_enthalpy_jax(self.gas, h, Y, t_guess,
+                                              pointwise=pointwise)
+        else:
+            T = temperature_from_enthalpy(self.gas, h, Y, t_guess,
+                                          extra=extra)
         rho = self.gas.get_density(p, T, Y)
         w = self.gas.get_net_production_rates(rho, T, Y)
         S = np.empty(phi.shape, dtype=np.result_type(phi, T))
@@ -225,7 +283,7 @@ class FlameletEquations:
 
     def rhs(self, phi, chi, vd, t_guess, p, h_ox, h_fu):
         """R = 1/2 chi Lap(phi) + S mask - phi_BC (equations.py:167-226)."""
-        S, _ = self.source_terms(phi, vd, t_guess, p)
+        S, _ = self.source_terms(phi, vd, t_guess, p, pointwise=False)
         return (0.5 * chi * apply_laplacian(self.abc, phi) + S * self.mask
                 - self.boundary_conditions(h_ox, h_fu))
 
@@ -288,8 +346,8 @@ def assemble_dense(lower, central, upper):
 # solver (solver.py:63-600)
 # ---------------------------------------------------------------------------
 class FlameletSolver:
-    def __init__(self, mech, x, y_ox, y_fu):
-        self.eqns = FlameletEquations(mech, x, y_ox, y_fu)
+    def __init__(self, mech, x, y_ox, y_fu, semantics="exact"):
+        self.eqns = FlameletEquations(mech, x, y_ox, y_fu, semantics)
         self.nv = self.eqns.nv
         self.log = []
 
@@ -352,7 +410,7 @@ class FlameletSolver:
                 p, h_ox, h_fu)
             temp = temperature_from_enthalpy(self.eqns.gas, phi[0], phi[1:],
                                              t_guess)
-            self.log.append(("cn", step, err))
+            self.log.append(("cn", step, err, float(np.max(np.real(temp)))))
         return phi, temp
 
     def equilibrate(self, phi, t_guess, p, h_ox, h_fu, steps=24, dt0=1e-8,
