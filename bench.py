@@ -65,6 +65,10 @@ def parse():
                          "(3 launches); calibrates HBM byte counters")
     ap.add_argument("--flamelets", type=int, default=256)
     ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--newton-norm", default="reference",
+                    choices=("reference", "scaled"))
+    ap.add_argument("--cpu-flamelets", type=int, default=2,
+                    help="flamelets of the CPU (numpy) flamelet baseline")
     ap.add_argument("--flamelet-reps", type=int, default=3)
     ap.add_argument("--flamelet-seed", type=int, default=4)
     ap.add_argument("--no-flamelets", action="store_true")
@@ -102,17 +106,39 @@ def device_inputs(mech, n, seed, device):
     return T, p, Y
 
 
-def cpu_baseline(mech, seconds, seed):
-    """numpy oracle on chunks of 2^16 cells until ~``seconds`` of work."""
+def cpu_model():
+    """The host CPU's model name from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def host_cores():
+    """CPUs this process may run on (the box's share, not the machine's)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def _cpu_wdot_chunks(task):
+    """Worker: the numpy oracle's net production rates on ``nchunks`` chunks
+    of 2^16 cells drawn from ``seed``; returns (cells, seconds in ω̇)."""
     import numpy as np
     from oracle.thermochem_np import Thermochemistry
+    from pyrometheus_amd import bundled
     from pyrometheus_amd.mechanism import ONE_ATM
+    name, seed, nchunks, chunk = task
+    mech = bundled(name)
     gas = Thermochemistry(mech)
     rng = np.random.default_rng(seed)
-    chunk = 1 << 16
-    done, t0 = 0, time.perf_counter()
-    elapsed = 0.0
-    while elapsed < seconds:
+    done, elapsed = 0, 0.0
+    for _ in range(nchunks):
         T = rng.uniform(300.0, 2500.0, chunk)
         p = ONE_ATM * np.exp(rng.uniform(np.log(0.5), np.log(20.0), chunk))
         Y = 0.5 * rng.dirichlet(0.5 * np.ones(mech.n_species), chunk).T
@@ -122,13 +148,96 @@ def cpu_baseline(mech, seconds, seed):
         gas.get_net_production_rates(rho, T, Y)
         elapsed += time.perf_counter() - t1
         done += chunk
-    return {"value": done / elapsed / 1e6, "unit": "Mcells/s", "cores": 1,
+    return done, elapsed
+
+
+def cpu_baseline(mech, seconds, seed, flamelet_probs=None):
+    """The numpy oracle (the reference's op structure: one whole-array ufunc
+    per generated term) on this host: 1 process on chunks of 2^16 cells
+    until ~``seconds`` of work, then a process pool over every available
+    core on as many chunks per process; optionally the oracle's flamelet
+    solve on a sample of the benchmark's flamelets (1 core, and a pool)."""
+    import multiprocessing as mp
+    chunk = 1 << 16
+    cores = host_cores()
+    cells, el = 0, 0.0
+    nch = 0
+    while el < seconds:
+        c, e = _cpu_wdot_chunks((mech.name, seed + nch, 1, chunk))
+        cells, el, nch = cells + c, el + e, nch + 1
+    one = cells / el / 1e6
+    per = max(1, nch // 2)
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_cpu_wdot_chunks, [(mech.name, 1000 + seed + j, per,
+                                           chunk) for j in range(cores)])
+    wall = time.perf_counter() - t0
+    pool_cells = sum(r[0] for r in res)
+    out = {"value": one, "unit": "Mcells/s", "cores": 1, "kind": "port",
+           "sample": f"{cells} cells of {mech.name} in chunks of {chunk} "
+                     "(seeded like the workload), numpy restatement of the "
+                     "generated code (whole-array ufunc per term), 1 "
+                     f"process; CPU {cpu_model()}; JAX absent",
+           "all_cores": {"value": pool_cells / wall / 1e6,
+                         "unit": "Mcells/s", "cores": cores,
+                         "sample": f"{pool_cells} cells, {per} chunks per "
+                                   f"process, {cores} processes (wall time "
+                                   "including process start)"},
+           "cpu_model": cpu_model()}
+    if flamelet_probs:
+        out["flamelets"] = cpu_flamelet_baseline(flamelet_probs, cores)
+    return out
+
+
+def _cpu_flamelet(prob):
+    """Worker: the numpy oracle's solve of one benchmark flamelet (the same
+    equilibration phase and hybrid solve as the device); ``prob`` is the
+    tuple (x, chi, phi, t_guess, y_ox, y_fu, h_ox, h_fu, pressure); returns
+    (seconds, converged, N)."""
+    import numpy as np
+    from oracle import flamelet_np as F
+    from pyrometheus_amd.mechanism import bundled
+    x, chi, phi, tg, y_ox, y_fu, h_ox, h_fu, p = prob
+    mech = bundled("sandiego")
+    t0 = time.perf_counter()
+    s = F.FlameletSolver(mech, x, y_ox, y_fu)
+    phi, T = s.equilibrate(phi, tg, p, h_ox, h_fu)
+    _, _, ok = s.solve(10, 1e-8, 20, 1e-9, 1e-5, 10, False, 10, chi,
+                       np.zeros(len(x)), T, p, h_ox, h_fu, phi)
+    return time.perf_counter() - t0, bool(ok), len(x)
+
+
+def cpu_flamelet_baseline(probs, cores):
+    """Flamelet solves/s of the numpy oracle: the given problems one after
+    another in one process, and ``cores`` copies of them spread over a
+    process pool."""
+    import multiprocessing as mp
+    import numpy as np
+    probs = [(p.x, p.chi, p.phi, p.t_guess, p.y_ox, p.y_fu, p.h_ox, p.h_fu,
+              p.pressure) for p in probs]
+    one = [_cpu_flamelet(p) for p in probs]
+    t_one = sum(r[0] for r in one)
+    ctx = mp.get_context("spawn")
+    batch = [probs[j % len(probs)] for j in range(cores)]
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_cpu_flamelet, batch)
+    wall = time.perf_counter() - t0
+    return {"value": len(one) / t_one, "unit": "solves/s", "cores": 1,
             "kind": "port",
-            "sample": f"{done} cells of {mech.name} in chunks of {chunk}, "
-                      f"numpy restatement of the generated code (whole-array "
-                      f"ufunc per term), 1 process, CPU "
-                      f"{platform.processor() or platform.machine()}, "
-                      f"{os.cpu_count()} host threads visible; JAX absent"}
+            "mean_n": float(np.mean([r[2] for r in one])),
+            "sample": f"{len(one)} of the benchmark's flamelets (N = "
+                      f"{[r[2] for r in one]}), size-stratified, numpy "
+                      "restatement of the flamelet solver (complex-step "
+                      "source Jacobian, LU block Thomas), equilibration "
+                      "and hybrid solve as on the device",
+            "converged": f"{sum(r[1] for r in one)}/{len(one)}",
+            "all_cores": {"value": len(batch) / wall, "unit": "solves/s",
+                          "cores": cores,
+                          "sample": f"{len(batch)} solves over {cores} "
+                                    "processes (wall time including process "
+                                    "start)"}}
 
 
 def pointwise_leg(args, device):
@@ -179,10 +288,24 @@ def pointwise_leg(args, device):
         8 * (ns + 2))
     ms_t = timed(lambda: gas.get_temperature(e, tg, Y, True))
     t_out = gas.get_temperature(e, tg, Y, True)
+    # GRI-3.0's e(T) is not continuous at T_mid: a target from within a
+    # band around T_mid may have a second root, which Newton may reach
+    t_mid, _, width = gas.get_temperature_jump_bands(Y)
+    near = ((T[None, :] - t_mid[:, None]).abs() <= 2.0 * width + 1e-12).any(0)
+    err = (t_out - T).abs()
+    resid = ((gas.get_mixture_internal_energy_mass(t_out, Y) - e).abs()
+             / gas.get_mixture_specific_heat_cv_mass(t_out, Y))
     res["temperature_from_energy"] = dict(
         entry(ms_t, n, 8 * (ns + 3)),
-        max_abs_err_K=float((t_out - T).abs().max().item()))
-    del e, tg, t_out
+        max_abs_err_K_outside_bands=float(err[~near].max().item()),
+        cells_in_bands=int(near.sum().item()),
+        max_abs_err_K_in_bands=(float(err[near].max().item())
+                                if bool(near.any()) else 0.0),
+        max_root_residual_K=float(resid.max().item()),
+        max_band_half_width_K=float(width.max().item()),
+        bands="cells within twice |e(T_mid+) - e(T_mid)|/c_v of a T_mid "
+              "(GRI-3.0: e drops by 2e-5..5e-4 K c_v at 1000 K, two roots)")
+    del e, tg, t_out, near, err, resid, width
     torch.cuda.empty_cache()
     m = n // 4
     Tm, Ym = T[:m].contiguous(), Y[:, :m].contiguous()
@@ -293,7 +416,7 @@ def flamelet_leg(args, device):
     gas = get_thermochem_class(mech)(device=device)
     doms, probs, chis = flamelet_batch(gas, n_flamelets=args.flamelets,
                                        seed=args.flamelet_seed)
-    kw = dict(equilibrate_steps=24, newton_norm="scaled")
+    kw = dict(equilibrate_steps=24, newton_norm=args.newton_norm)
     sizes = np.array([len(p.x) for p in probs])
     # block-Thomas schedule for nv = 10 on this device, tuned on this
     # workload's flamelet sizes (synthetic blocks)
@@ -331,11 +454,17 @@ def flamelet_leg(args, device):
             else "pyro_flamelet_thomas")
     iters = len(events[tkey])
     lat_b = np.array([r.finished_at for r in res])
+    resid = max_scaled_residual(gas, probs, res)
     plain_t, plain_ok = [], 0
-    for j in range(min(args.plain_sample, len(probs))):
+    # size-stratified sample: the flamelets at evenly spaced ranks of N
+    order = np.argsort(sizes, kind="stable")
+    k = min(args.plain_sample, len(probs))
+    sample = order[np.round(np.linspace(0, len(probs) - 1, k)).astype(int)]
+    for j in sample:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        _, _, ok = solve_plain(gas, doms[j], probs[j], **kw)
+        _, _, ok = solve_plain(gas, doms[j], probs[j], semantics="exact",
+                               **kw)
         torch.cuda.synchronize()
         plain_t.append(time.perf_counter() - t0)
         plain_ok += bool(ok)
@@ -351,7 +480,14 @@ def flamelet_leg(args, device):
     # block row it reads C (nv^2), rhs (nv), L and U, writes x (nv)
     bytes_row = 8 * (nv * nv + 2 * nv + 2)
     flop_row = 4.67 * nv ** 3 + 8 * nv ** 2
-    return {
+    # the CPU flamelet baseline's sample (bounded CPU time): flamelets at
+    # the 5 % .. 25 % ranks of N; its result is also scaled to the batch's
+    # mean N (CPU time per solve grows about linearly with N)
+    kc = min(args.cpu_flamelets, len(probs))
+    cpu_sample = [probs[j] for j in order[np.round(
+        np.linspace(0.05, 0.25, kc) * (len(probs) - 1)).astype(int)]]
+    args.batch_mean_n = float(sizes.mean())
+    return cpu_sample, {
         "workload": f"{len(probs)} steady flamelets, {mech.name} "
                     f"({mech.n_species} species), N in [64, 1024] "
                     "Z-clustered, chi_st log-uniform [1, 1000] 1/s, "
@@ -368,11 +504,19 @@ def flamelet_leg(args, device):
         "batched_latency_p99_s": float(np.percentile(lat_b, 99)),
         "plain_latency_p50_s": float(np.percentile(plain_t, 50)),
         "plain_latency_p99_s": float(np.percentile(plain_t, 99)),
-        "plain_sample": f"first {len(plain_t)} of the same problems, one "
-                        "FlameletSolver each, sequential",
+        "plain_sample": f"{len(plain_t)} of the same problems at evenly "
+                        "spaced ranks of N (mean N "
+                        f"{sizes[sample].mean():.0f}, batch mean "
+                        f"{sizes.mean():.0f}), one FlameletSolver each "
+                        "(same iteration as the batch: exact source "
+                        "Jacobian), sequential",
+        "residual": resid,
         "converged_batched": f"{sum(r.converged for r in res)}/{len(res)}",
         "converged_plain": f"{plain_ok}/{len(plain_t)}",
-        "newton_norm": "scaled (enthalpy row / max(|h_ox|, |h_fu|))",
+        "newton_norm": ("reference (|v|_2 over every row, "
+                        "solver.py:136-139) < 1e-8"
+                        if args.newton_norm == "reference" else
+                        "scaled (enthalpy row / max(|h_ox|, |h_fu|)) < 1e-8"),
         "batch_iterations": iters,
         "kernel_ms": kms,
         "thomas_roofline": {
@@ -388,6 +532,104 @@ def flamelet_leg(args, device):
                     "two wavefronts each, interface system, recovery: "
                     "latency-bound chains"},
     }
+
+
+def max_scaled_residual(gas, probs, results):
+    """Converged residual of every flamelet against its gross scale: for the
+    enthalpy row and for the mass-fraction rows together, max |R| over
+    max (|chi/2| (|a phi_-| + |b phi| + |c phi_+|) + |S|) of that flamelet,
+    R and S evaluated by the device kernels at the returned state and
+    temperature; returns the worst over the batch and the count above
+    1e-10."""
+    import numpy as np
+    import torch
+    from pyrometheus_amd.flamelets.engine import FlameletBatch
+    b = FlameletBatch(gas, [p.x for p in probs],
+                      np.stack([p.y_ox for p in probs]),
+                      np.stack([p.y_fu for p in probs]))
+    b.set_boundary_enthalpy([p.h_ox for p in probs],
+                            [p.h_fu for p in probs])
+    dev = b.dev
+    phi = torch.cat([torch.cat([r.state.enthalpy[None],
+                                r.state.mass_fractions]) for r in results],
+                    dim=1).contiguous()
+    T = torch.cat([r.temperature for r in results]).contiguous()
+    chi = torch.as_tensor(np.concatenate([p.chi for p in probs]),
+                          dtype=torch.float64, device=dev)
+    pres = b.per_point([p.pressure for p in probs])
+    vd = torch.zeros(b.P, dtype=torch.float64, device=dev)
+    R = torch.empty_like(phi)
+    b.residual(phi, chi, vd, T, pres, R)
+    S = b.S.clone()
+    ap = phi.abs()
+    gross = b.lb.abs() * ap
+    nxt = torch.zeros_like(ap)
+    prv = torch.zeros_like(ap)
+    prv[:, 1:] = ap[:, :-1]
+    nxt[:, :-1] = ap[:, 1:]
+    gross = 0.5 * chi.abs() * (gross + b.la.abs() * prv + b.lc.abs() * nxt)
+    gross += S.abs()
+    worst = []
+    for a, e in zip(b.offsets_np[:-1], b.offsets_np[1:]):
+        r, g = R[:, a:e].abs(), gross[:, a:e]
+        worst.append(max(float(r[0].max() / g[0].max()),
+                         float(r[1:].max() / g[1:].max())))
+    worst = np.array(worst)
+    return {"max_scaled": float(worst.max()),
+            "above_1e-10": int((worst > 1e-10).sum()),
+            "flamelets": len(worst),
+            "definition": "per flamelet, enthalpy row and mass-fraction rows "
+                          "separately: max|R| / max(|chi/2|(|a phi_-| + "
+                          "|b phi| + |c phi_+|) + |S|)"}
+
+
+def config1_leg(args, device):
+    """The CPU-runnable configuration on the device: net production rates
+    and T from e (guesses T +- 300 K) of the 9-species synthetic H2-like
+    mechanism (21 reactions: 3 Troe, 4 three-body) over 65,536 cells,
+    seed 1."""
+    import torch
+    from pyrometheus_amd import bundled, get_thermochem_class
+    from pyrometheus_amd.flops import wdot_flops
+    mech = bundled("synth9")
+    gas = get_thermochem_class(mech)(device=device)
+    n = 1 << 16
+    T, p, Y = device_inputs(mech, n, 1, device)
+    rho = gas.get_density(p, T, Y)
+    g = torch.Generator(device=device).manual_seed(11)
+    tg = (T + 600.0 * (torch.rand(n, generator=g, dtype=torch.float64,
+                                  device=device) - 0.5)).clamp(200.0, 3500.0)
+    e = gas.get_mixture_internal_energy_mass(T, Y)
+    out = torch.empty((mech.n_species, n), dtype=torch.float64, device=device)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, reps=20):
+        fn()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True),
+               torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for a, b in ev:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return sum(a.elapsed_time(b) for a, b in ev) / reps
+    ms_w = timed(lambda: gas.get_net_production_rates(rho, T, Y, out=out))
+    ms_t = timed(lambda: gas.get_temperature(e, tg, Y, True))
+    t_out = gas.get_temperature(e, tg, Y, True)
+    fl = wdot_flops(mech)
+    return {"workload": f"synth9 ({mech.n_species} species, "
+                        f"{mech.n_reactions} reactions), {n} cells (seed 1)",
+            "wdot_ms": ms_w, "wdot_mcells_s": n / ms_w / 1e3,
+            "wdot_fp64_frac": fl["flop_per_cell"] * n / (ms_w * 1e-3)
+            / 1e12 / FP64_PEAK_TFLOPS,
+            "wdot_hbm_frac": fl["bytes_per_cell"] * n / (ms_w * 1e-3) / 1e9
+            / HBM_PEAK_GBS,
+            "temperature_ms": ms_t,
+            "temperature_mcells_s": n / ms_t / 1e3,
+            "temperature_max_abs_err_K": float((t_out - T).abs().max()),
+            "note": "65,536 cells fill 256 of the chip's 1,024 SIMDs "
+                    "with one wave each: launch- and latency-bound"}
 
 
 def serving_leg(args, device):
@@ -473,7 +715,7 @@ def main():
         if not args.no_derivatives:
             line["derivatives"] = derivatives_leg(args, device)
         if not args.no_flamelets:
-            line["flamelets"] = flamelet_leg(args, device)
+            _, line["flamelets"] = flamelet_leg(args, device)
         if not args.no_serving:
             line["serving"] = serving_leg(args, device)
         print(json.dumps(line), flush=True)
@@ -534,7 +776,7 @@ def main():
                                f"{mech.n_reactions} reactions), {n} cells, "
                                "SoA FP64 state resident in HBM",
                    "cells": n, "mechanism": mech.name},
-        "roofline": {"bound": "mfma", "achieved": achieved,
+        "roofline": {"bound": "fp64-valu", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic,
@@ -554,13 +796,26 @@ def main():
         line["pointwise"] = pointwise_leg(args, device)
     if not args.no_derivatives:
         line["derivatives"] = derivatives_leg(args, device)
+    line["config1"] = config1_leg(args, device)
+    cpu_probs = None
     if not args.no_flamelets:
-        line["flamelets"] = flamelet_leg(args, device)
+        cpu_probs, line["flamelets"] = flamelet_leg(args, device)
     if not args.no_serving:
         line["serving"] = serving_leg(args, device)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(mech, args.cpu_seconds, args.seed)
+        line["cpu_baseline"] = cpu_baseline(mech, args.cpu_seconds, args.seed,
+                                            cpu_probs)
+        fl = line["cpu_baseline"].get("flamelets")
+        if fl and getattr(args, "batch_mean_n", None):
+            f = fl["mean_n"] / args.batch_mean_n
+            fl["value_at_batch_mean_n"] = fl["value"] * f
+            fl["all_cores"]["value_at_batch_mean_n"] = (
+                fl["all_cores"]["value"] * f)
+        c1, e1 = _cpu_wdot_chunks(("synth9", 1, 4, 1 << 16))
+        line["cpu_baseline"]["config1"] = {
+            "value": c1 / e1 / 1e6, "unit": "Mcells/s", "cores": 1,
+            "sample": "4 chunks of 65,536 synth9 cells, numpy oracle"}
     print(json.dumps(line), flush=True)
 
 

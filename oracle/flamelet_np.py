@@ -276,7 +276,7 @@ class FlameletEquations:
                                           extra=extra)
         rho = self.gas.get_density(p, T, Y)
         w = self.gas.get_net_production_rates(rho, T, Y)
-        S = np.empty(phi.shape, dtype=np.result_type(phi, T))
+        S = np.empty(phi.shape, dtype=np.result_type(phi, T, rho))
         S[0] = vd / rho
         S[1:] = self.gas.molecular_weights[:, None] * w / rho
         return S, T

@@ -151,10 +151,15 @@ class NetProductionRates(torch.autograd.Function):
         rho, temperature, y = ctx.saved_tensors
         g_r, g_t, g_y = wdot_vjp(ctx.gas, rho, temperature, y,
                                  wbar.contiguous())
+        cells = tuple(y.shape[1:])
+
+        def reduce(g, like):
+            # per-cell gradient back to the input's shape: a scalar or a
+            # broadcast input gets the sum over the cells it reached
+            return g.reshape(cells).sum_to_size(like.shape)
         return (None,
-                g_r.reshape(rho.shape) if ctx.needs_input_grad[1] else None,
-                g_t.reshape(temperature.shape)
-                if ctx.needs_input_grad[2] else None,
+                reduce(g_r, rho) if ctx.needs_input_grad[1] else None,
+                reduce(g_t, temperature) if ctx.needs_input_grad[2] else None,
                 g_y.reshape(y.shape) if ctx.needs_input_grad[3] else None)
 
     @staticmethod

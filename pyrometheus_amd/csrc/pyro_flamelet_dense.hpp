@@ -10,39 +10,15 @@
 
 namespace pyro_fl {
 
-// Newton on h(T, Y) = h: take the step, stop once |step| < tol.
-__device__ __forceinline__ bool temperature_from_h(double h,
-                                                   const double (&Y)[NS],
-                                                   double t, double tol,
-                                                   int maxit, double& T) {
-  for (int it = 0; it < maxit; ++it) {
-    double hrt[NS], cpr[NS];
-    mech::species_h_rt(t, 1.0 / t, hrt);
-    mech::species_cp_r(t, cpr);
-    double sh = 0.0, sc = 0.0;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      const double yw = Y[k] * mech::kInvW[k];
-      sh = fma(yw, hrt[k], sh);
-      sc = fma(yw, cpr[k], sc);
-    }
-    const double step = (h - kGasConstant * t * sh) / (kGasConstant * sc);
-    t += step;
-    if (fabs(step) < tol) {
-      T = t;
-      return true;
-    }
-  }
-  T = t;
-  return false;
-}
-
 // Source term S = [vd/rho ; W_k w_k / rho] of one point
 // (flamelets/equations.py:313-389) and, if jout != nullptr, its Jacobian
 // dS/dphi (NV x NV, row-major) by the chain rule through T(h, Y) and
 // rho(T, Y) = p W/(R T):
 //   dT/dh = 1/cp, dT/dY_m = -h_m/cp, drho/dh = -rho/T dT/dh,
 //   drho/dY_m = -rho W/W_m - rho/T dT/dY_m, dC_k = (Y_k/W_k) drho + rho/W_k dY_k.
+// kRef: T and dT/dphi from the reference's loop (temperature_from_h_tangent)
+// instead of the converged Newton and the exact derivative of T(h, Y).
+template <bool kRef>
 __device__ __forceinline__ bool point_source(const double h,
                                              const double (&Y)[NS],
                                              const double vd,
@@ -51,7 +27,13 @@ __device__ __forceinline__ bool point_source(const double h,
                                              const int maxit, double& T,
                                              double (&S)[NV],
                                              double* __restrict__ jout) {
-  const bool ok = temperature_from_h(h, Y, tguess, tol, maxit, T);
+  double th_ref = 0.0, tY_ref[NS];
+  bool ok;
+  if constexpr (kRef)
+    ok = temperature_from_h_tangent(h, Y, tguess, tol, maxit, T, th_ref,
+                                    tY_ref);
+  else
+    ok = temperature_from_h(h, Y, tguess, tol, maxit, T);
   double s = 0.0;
 #pragma unroll
   for (int k = 0; k < NS; ++k) s = fma(mech::kInvW[k], Y[k], s);
@@ -73,13 +55,14 @@ __device__ __forceinline__ bool point_source(const double h,
 #pragma unroll
     for (int k = 0; k < NS; ++k) sc = fma(Y[k] * mech::kInvW[k], cpr[k], sc);
     const double cp = kGasConstant * sc;
-    const double dTdh = 1.0 / cp;
+    const double dTdh = kRef ? th_ref : 1.0 / cp;
     const double rT = rho / T;
     const double drdh = -rT * dTdh;
     double dTdY[NS], drdY[NS];
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
-      dTdY[m] = -(kGasConstant * T * hrt[m] * mech::kInvW[m]) * dTdh;
+      dTdY[m] = kRef ? tY_ref[m]
+                     : -(kGasConstant * T * hrt[m] * mech::kInvW[m]) * dTdh;
       drdY[m] = fma(-rT, dTdY[m], -rho * mmw * mech::kInvW[m]);
     }
     // row 0: d(vd/rho)
@@ -111,6 +94,7 @@ __device__ __forceinline__ bool point_source(const double h,
   return ok;
 }
 
+template <bool kRef>
 __global__ void __launch_bounds__(256)
 k_flamelet_source(const double* __restrict__ phi, int64_t ldp,
                   const double* __restrict__ vd,
@@ -125,12 +109,16 @@ k_flamelet_source(const double* __restrict__ phi, int64_t ldp,
   const double h = phi[i];
 #pragma unroll
   for (int k = 0; k < NS; ++k) Y[k] = phi[(1 + k) * ldp + i];
-  const bool ok = point_source(h, Y, vd ? vd[i] : 0.0, tguess[i * tg_stride],
+  const bool ok = point_source<kRef>(h, Y, vd ? vd[i] : 0.0,
+                                     tguess[i * tg_stride],
                                p[i * p_stride], tol, maxit, T, s,
                                J ? J + i * NV * NV : nullptr);
+  // a point whose temperature did not converge poisons its flamelet's
+  // update (NaN S), which the caller sees in the update norm
+  const double bad = __builtin_nan("");
 #pragma unroll
-  for (int v = 0; v < NV; ++v) S[v * lds + i] = s[v];
-  if (T_out) T_out[i] = T;
+  for (int v = 0; v < NV; ++v) S[v * lds + i] = ok ? s[v] : bad;
+  if (T_out) T_out[i] = ok ? T : bad;
   if (!ok && n_fail) atomicAdd(n_fail, 1);
 }
 
@@ -142,13 +130,21 @@ PYRO_API int pyro_flamelet_source(const double* phi, int64_t ldp,
                                   int64_t p_stride, double* S, int64_t lds,
                                   double* T_out, double* J, int64_t n,
                                   double tol, int maxit, int* n_fail,
-                                  hipStream_t stream) {
+                                  int semantics, hipStream_t stream) {
   if (n == 0) return pyro::kOk;
-  if (n < 0 || ldp < n || lds < n || !phi || !tguess || !p || !S)
+  if (n < 0 || ldp < n || lds < n || !phi || !tguess || !p || !S ||
+      (semantics != 0 && semantics != 1))
     return pyro::kInvalidArgument;
-  pyro_fl::k_flamelet_source<<<pyro::grid_for(n, 256), 256, 0, stream>>>(
-      phi, ldp, vd, tguess, tg_stride, p, p_stride, S, lds, T_out, J, n, tol,
-      maxit, n_fail);
+  if (semantics == 1)
+    pyro_fl::k_flamelet_source<true><<<pyro::grid_for(n, 256), 256, 0,
+                                       stream>>>(
+        phi, ldp, vd, tguess, tg_stride, p, p_stride, S, lds, T_out, J, n,
+        tol, maxit, n_fail);
+  else
+    pyro_fl::k_flamelet_source<false><<<pyro::grid_for(n, 256), 256, 0,
+                                        stream>>>(
+        phi, ldp, vd, tguess, tg_stride, p, p_stride, S, lds, T_out, J, n,
+        tol, maxit, n_fail);
   return pyro::launch_status();
 }
 

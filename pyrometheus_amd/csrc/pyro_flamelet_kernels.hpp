@@ -20,11 +20,94 @@ using namespace pyro;
 constexpr int NS = mech::NS;
 constexpr int NV = NS + 1;
 
+// Newton on h(T, Y) = h: take the step, stop once |step| < tol.
+__device__ __forceinline__ bool temperature_from_h(double h,
+                                                   const double (&Y)[NS],
+                                                   double t, double tol,
+                                                   int maxit, double& T) {
+  for (int it = 0; it < maxit; ++it) {
+    double hrt[NS], cpr[NS];
+    mech::species_h_rt(t, 1.0 / t, hrt);
+    mech::species_cp_r(t, cpr);
+    double sh = 0.0, sc = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const double yw = Y[k] * mech::kInvW[k];
+      sh = fma(yw, hrt[k], sh);
+      sc = fma(yw, cpr[k], sc);
+    }
+    const double step = (h - kGasConstant * t * sh) / (kGasConstant * sc);
+    t += step;
+    if (fabs(step) < tol) {
+      T = t;
+      return true;
+    }
+  }
+  T = t;
+  return false;
+}
+
+// Newton on h(T, Y) = h as the reference's JAX loop runs it
+// (flamelets/make_pyro.py:255-299, a lax.while_loop): the step is computed,
+// and the loop stops before taking it once |step| <= tol.  Alongside, the
+// forward-mode tangents th = dT/dh and tY[m] = dT/dY_m are carried through
+// the iterations actually taken -- what jax.jacfwd through that loop
+// differentiates (equations.py:124-129).  With step = (h - H)/Cp,
+// H = R T sum yw_k h_k/RT, Cp = R sum yw_k cp_k/R, one iteration maps
+//   th    <- 1/Cp - step dCp/dT th / Cp
+//   tY[m] <- -(R T h_m/RT / W_m)/Cp - step (dCp/dT tY[m] + R cp_m/R / W_m)/Cp;
+// a guess that already meets the test takes no iteration and leaves the
+// tangents 0 (the reference's Jacobian then has no dT/dphi term).
+__device__ __forceinline__ bool temperature_from_h_tangent(
+    double h, const double (&Y)[NS], double t, double tol, int maxit,
+    double& T, double& th, double (&tY)[NS]) {
+  th = 0.0;
+#pragma unroll
+  for (int m = 0; m < NS; ++m) tY[m] = 0.0;
+  for (int it = 0; it < maxit; ++it) {
+    double hrt[NS], cpr[NS], dcp[NS];
+    mech::species_h_rt(t, 1.0 / t, hrt);
+    mech::species_cp_r(t, cpr);
+    mech::species_dcp_r(t, dcp);
+    double sh = 0.0, sc = 0.0, sd = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const double yw = Y[k] * mech::kInvW[k];
+      sh = fma(yw, hrt[k], sh);
+      sc = fma(yw, cpr[k], sc);
+      sd = fma(yw, dcp[k], sd);
+    }
+    const double cp = kGasConstant * sc;
+    const double step = (h - kGasConstant * t * sh) / cp;
+    if (!(fabs(step) > tol)) {
+      T = t;
+      return true;
+    }
+    const double icp = 1.0 / cp;
+    const double sdc = step * kGasConstant * sd * icp;  // step dCp/dT / Cp
+    th = fma(-sdc, th, icp);
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      const double dh = kGasConstant * t * hrt[m] * mech::kInvW[m];
+      const double dc = kGasConstant * cpr[m] * mech::kInvW[m];
+      tY[m] = fma(-sdc, tY[m], -(dh + step * dc) * icp);
+    }
+    t += step;
+  }
+  T = t;
+  return false;
+}
+
+
 // Residual and block Jacobian of the steady (mode 0) or Crank-Nicolson
 // (mode 1) flamelet equations for a batch of flamelets whose points are
 // concatenated; point i belongs to system sys[i] = s with rows
 // [off[s], off[s+1]).  (flamelets/equations.py:167-282, solver.py:253-306)
 //   R_i = chi_i/2 (a_i phi_{i-1} + b_i phi_i + c_i phi_{i+1}) + m_i S_i - BC_i
+// evaluated on interior points as a_i (phi_{i-1} - phi_i) + c_i (phi_{i+1} -
+// phi_i) (b_i = -(a_i + c_i)): the neighbour differences are exact for a
+// smooth profile, so the enthalpy row (h ~ 1e5 J/kg, linear in Z, exact
+// Laplacian 0) keeps a rounding floor set by the differences, not by h.
 //   C_i = chi_i/2 b_i I + m_i J_i,  L_i = chi_{i+1}/2 a_{i+1},  U_i = chi_i/2 c_i
 // CN: R' = phi - phi_prev - dt/2 (R + R_prev), C' = I - dt/2 C, L' = -dt/2 L,
 // U' = -dt/2 U.  The interior mask m_i is 0 on the two end points.
@@ -67,9 +150,13 @@ k_flamelet_assemble(const double* __restrict__ phi, int64_t ldp,
   const double hdt = (mode == 1) ? 0.5 * dt[s] : 0.0;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
-    double lap = bi * phi[v * ldp + i];
-    if (!first) lap = fma(ai, phi[v * ldp + i - 1], lap);
-    if (!last) lap = fma(ci, phi[v * ldp + i + 1], lap);
+    const double p0 = phi[v * ldp + i];
+    double lap;
+    if (first || last)
+      lap = bi * p0;
+    else
+      lap = fma(ai, phi[v * ldp + i - 1] - p0,
+                ci * (phi[v * ldp + i + 1] - p0));
     double r = fma(hc, lap, mask * S[v * lds + i]);
     if (first) r -= bc[(s * 2 + 0) * NV + v];
     if (last) r -= bc[(s * 2 + 1) * NV + v];
@@ -178,6 +265,105 @@ k_system_norm2(const double* __restrict__ x, const int64_t* __restrict__ off,
   }
 }
 
+// R T (R = R_u / W, the specific gas constant) of every point and its
+// gradient with respect to phi = [h; Y] (equations.py:518-535 and the jacfwd
+// of it, :140-149), for the compressible-EOS adjoint solves.  T(h, Y) by
+// Newton from tguess: kRef the reference's loop with its forward-mode
+// tangent (temperature_from_h_tangent), otherwise the converged T and the
+// exact dT/dh = 1/cp, dT/dY_m = -h_m/cp.  rtj is (NV, ldj):
+//   d(RT)/dh = R dT/dh,  d(RT)/dY_m = R_u T / W_m + R dT/dY_m.
+// A point whose Newton did not converge gets NaN.
+template <bool kRef>
+__global__ void __launch_bounds__(256)
+k_flamelet_eos_rt(const double* __restrict__ phi, int64_t ldp,
+                  const double* __restrict__ tguess, int64_t tg_stride,
+                  double* __restrict__ rt, double* __restrict__ rtj,
+                  int64_t ldj, int64_t n, double tol, int maxit,
+                  int* __restrict__ n_fail) {
+  const int64_t i = cell_index();
+  if (i >= n) return;
+  double Y[NS];
+  const double h = phi[i];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) Y[k] = phi[(1 + k) * ldp + i];
+  double T, th, tY[NS];
+  bool ok;
+  if constexpr (kRef) {
+    ok = temperature_from_h_tangent(h, Y, tguess[i * tg_stride], tol, maxit,
+                                    T, th, tY);
+  } else {
+    ok = temperature_from_h(h, Y, tguess[i * tg_stride], tol, maxit, T);
+    double hrt[NS], cpr[NS];
+    mech::species_h_rt(T, 1.0 / T, hrt);
+    mech::species_cp_r(T, cpr);
+    double sc = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) sc = fma(Y[k] * mech::kInvW[k], cpr[k], sc);
+    th = 1.0 / (kGasConstant * sc);
+#pragma unroll
+    for (int m = 0; m < NS; ++m)
+      tY[m] = -(kGasConstant * T * hrt[m] * mech::kInvW[m]) * th;
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) s = fma(mech::kInvW[k], Y[k], s);
+  const double rs = kGasConstant * s;  // R = R_u / W
+  const double bad = __builtin_nan("");
+  rt[i] = ok ? rs * T : bad;
+  if (rtj) {
+    rtj[i] = ok ? rs * th : bad;
+#pragma unroll
+    for (int m = 0; m < NS; ++m)
+      rtj[(1 + m) * ldj + i] =
+          ok ? fma(rs, tY[m], kGasConstant * T * mech::kInvW[m]) : bad;
+  }
+  if (!ok && n_fail) atomicAdd(n_fail, 1);
+}
+
+// The adjoint operator of the steady flamelet Jacobian (equations.py:
+// 451-516) in place of the source-Jacobian blocks J (P, NV, NV): central
+// blocks b_i I + 2 J_i^T / chi_i on interior points and I on the two ends
+// of each system, the Laplacian's own couplings L_i (row i+1 to x_i) =
+// a_{i+1} where row i+1 is interior and U_i (row i to x_{i+1}) = c_i where
+// row i is interior (0 elsewhere).  One thread per block entry; the thread
+// of (r, c) with r <= c writes both (r, c) and (c, r).
+__global__ void __launch_bounds__(256)
+k_flamelet_adjoint_blocks(double* __restrict__ J,
+                          const double* __restrict__ chi,
+                          const double* __restrict__ la,
+                          const double* __restrict__ lb,
+                          const double* __restrict__ lc,
+                          const int* __restrict__ sys,
+                          const int64_t* __restrict__ off,
+                          double* __restrict__ L, double* __restrict__ U,
+                          int64_t n) {
+  constexpr int64_t NB = static_cast<int64_t>(NV) * NV;
+  const int64_t f = cell_index();
+  if (f >= n * NB) return;
+  const int64_t i = f / NB;
+  const int e = static_cast<int>(f - i * NB);
+  const int r = e / NV, c = e - r * NV;
+  const int sy = sys[i];
+  const bool first = (i == off[sy]);
+  const bool last = (i == off[sy + 1] - 1);
+  if (e == 0) {
+    L[i] = (!last && i + 1 < off[sy + 1] - 1) ? la[i + 1] : 0.0;
+    U[i] = (!first && !last) ? lc[i] : 0.0;
+  }
+  if (r > c) return;
+  double* blk = J + i * NB;
+  const double d = (r == c) ? 1.0 : 0.0;
+  if (first || last) {
+    blk[r * NV + c] = d;
+    blk[c * NV + r] = d;
+    return;
+  }
+  const double s = 2.0 / chi[i];
+  const double jrc = blk[r * NV + c], jcr = blk[c * NV + r];
+  blk[r * NV + c] = fma(s, jcr, d * lb[i]);
+  if (r != c) blk[c * NV + r] = s * jrc;
+}
+
 }  // namespace pyro_fl
 
 // ---------------------------------------------------------------------------
@@ -267,4 +453,42 @@ PYRO_API int pyro_flamelet_thomas_partitioned(
         L, C, U, R, X, off, pstart, pend, psys, nparts, jbefore, jafter,
         roff, nsys, nj, fact, maps, red, Z, n_singular, stream);
   }
+}
+
+PYRO_API int pyro_flamelet_eos_rt(const double* phi, int64_t ldp,
+                                  const double* tguess, int64_t tg_stride,
+                                  double* rt, double* rtj, int64_t ldj,
+                                  int64_t n, double tol, int maxit,
+                                  int* n_fail, int semantics,
+                                  hipStream_t stream) {
+  if (n == 0) return pyro::kOk;
+  if (n < 0 || ldp < n || !phi || !tguess || !rt || (rtj && ldj < n) ||
+      (semantics != 0 && semantics != 1))
+    return pyro::kInvalidArgument;
+  if (semantics == 1)
+    pyro_fl::k_flamelet_eos_rt<true><<<pyro::grid_for(n, 256), 256, 0,
+                                       stream>>>(
+        phi, ldp, tguess, tg_stride, rt, rtj, ldj, n, tol, maxit, n_fail);
+  else
+    pyro_fl::k_flamelet_eos_rt<false><<<pyro::grid_for(n, 256), 256, 0,
+                                        stream>>>(
+        phi, ldp, tguess, tg_stride, rt, rtj, ldj, n, tol, maxit, n_fail);
+  return pyro::launch_status();
+}
+
+PYRO_API int pyro_flamelet_adjoint_blocks(double* J, const double* chi,
+                                          const double* la, const double* lb,
+                                          const double* lc, const int* sys,
+                                          const int64_t* off, double* L,
+                                          double* U, int64_t n,
+                                          hipStream_t stream) {
+  if (n == 0) return pyro::kOk;
+  if (n < 0 || !J || !chi || !la || !lb || !lc || !sys || !off || !L || !U)
+    return pyro::kInvalidArgument;
+  pyro_fl::k_flamelet_adjoint_blocks<<<pyro::grid_for(
+                                           n * pyro_fl::NV * pyro_fl::NV,
+                                           256),
+                                       256, 0, stream>>>(
+      J, chi, la, lb, lc, sys, off, L, U, n);
+  return pyro::launch_status();
 }

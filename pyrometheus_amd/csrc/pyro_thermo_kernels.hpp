@@ -375,7 +375,7 @@ k_temperature(const double* __restrict__ target, int64_t target_stride,
     }
   }
   T_out[i] = t;
-  if (iters) iters[i] = it;
+  if (iters) iters[i] = ok ? it : -1;
   if (!ok && n_fail) atomicAdd(n_fail, 1);
 }
 

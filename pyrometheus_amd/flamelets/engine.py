@@ -90,12 +90,23 @@ def _rows(t, name, writable=False):
 class FlameletBatch:
     def __init__(self, gas, grids: Sequence[np.ndarray], y_ox, y_fu,
                  device=None, jacobian: str = "auto",
-                 thomas: str = "auto", part_len: Optional[int] = None):
+                 thomas: str = "auto", part_len: Optional[int] = None,
+                 semantics: str = "exact"):
         lib = gas._lib
         if not lib.has("pyro_flamelet_assemble"):
             raise HipError(f"library of {gas._mech.name} has no flamelet "
                            "kernels")
         self.gas, self.lib = gas, lib
+        # "exact": T(h, Y) converged and its exact derivative in the source
+        # Jacobian; "reference": the reference's JAX loop and the tangent
+        # jacfwd carries through it (pyro_flamelet_source, semantics 1)
+        if semantics not in ("exact", "reference"):
+            raise ValueError(f"unknown semantics {semantics!r}")
+        if semantics == "reference" and (jacobian == "cached" or not lib.has(
+                "pyro_flamelet_source")):
+            raise HipError("reference semantics need the register-Jacobian "
+                           "source kernel (at most 16 species)")
+        self.semantics = semantics
         # one-pass register Jacobian (small mechanisms) or the two-pass
         # cached form (any mechanism); "auto" takes the first available
         if jacobian not in ("auto", "dense", "cached"):
@@ -106,6 +117,7 @@ class FlameletBatch:
         self.dense = (jacobian != "cached"
                       and lib.has("pyro_flamelet_source"))
         self._jac_work = None
+        self._iters = None
         self.dev = torch.device(device) if device is not None else gas.device
         self.ns = gas.num_species
         self.nv = self.ns + 1
@@ -240,15 +252,21 @@ class FlameletBatch:
                        _ptr(vd), tguess.data_ptr(), 1, p.data_ptr(), 1,
                        self.S.data_ptr(), self.P, self.T.data_ptr(),
                        self.J.data_ptr() if jac else None, self.P, tol, maxit,
-                       self.n_fail.data_ptr(), _stream())
+                       self.n_fail.data_ptr(),
+                       1 if self.semantics == "reference" else 0, _stream())
             return
         # T(h, Y): take the Newton step, stop once |step| < tol, as the
         # one-pass kernel does
         P, ldp = self.P, phi.stride(0)
         y_ptr = phi.data_ptr() + 8 * ldp
+        if self._iters is None:
+            self._iters = torch.empty(P, dtype=torch.int32, device=self.dev)
         self._call("pyro_temperature", phi.data_ptr(), 1, tguess.data_ptr(),
                    1, y_ptr, ldp, self.T.data_ptr(), P, 0, 0, tol, maxit,
-                   self.n_fail.data_ptr(), None, _stream())
+                   self.n_fail.data_ptr(), self._iters.data_ptr(), _stream())
+        # as the one-pass kernel does: a point whose temperature did not
+        # converge gets T = NaN, so its flamelet's update is not finite
+        self.T.masked_fill_(self._iters < 0, float("nan"))
         if jac:
             if self._jac_work is None:
                 need = int(self.lib.call("pyro_source_jacobian_work_doubles"))

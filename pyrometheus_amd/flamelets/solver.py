@@ -38,16 +38,30 @@ def _t(x, dev, n=None):
     return t.contiguous()
 
 
-class FlameletEquations:
-    """Residual and block Jacobian of the steady flamelet equations."""
+SEMANTICS = ("reference", "exact")
 
-    def __init__(self, pyro_gas, domain, y_ox, y_fu):
+
+class FlameletEquations:
+    """Residual and block Jacobian of the steady flamelet equations.
+
+    ``semantics="reference"`` recovers T(h, Y) with the reference's loop and
+    differentiates through it as ``jax.jacfwd`` does (``equations.py:
+    124-129`` over ``make_pyro.py:255-299``): where the temperature guess
+    already meets the loop's test, no iteration runs and the source Jacobian
+    has no dT/dphi term.  That is the case at the first Newton iteration
+    after a time march and at the first iteration of every Crank-Nicolson
+    step, so it shapes the reference's Newton iterates (not its converged
+    state).  ``"exact"`` uses the converged T and its exact derivative; it
+    needs no register-Jacobian kernel and serves mechanisms of any size."""
+
+    def __init__(self, pyro_gas, domain, y_ox, y_fu, semantics="exact"):
         self.pyro_gas = pyro_gas
         self.domain = domain
         self.mol_wts = pyro_gas.molecular_weights
         self.batch = FlameletBatch(pyro_gas, [domain.x],
                                    np.asarray(_cpu(y_ox)),
-                                   np.asarray(_cpu(y_fu)))
+                                   np.asarray(_cpu(y_fu)),
+                                   semantics=semantics)
         self.num_v = pyro_gas.num_species + 1
 
     def _args(self, state, diss_rate, viscous_diss, temp_guess, pressure):
@@ -149,18 +163,31 @@ def update_norm(sums, kind="reference", h_scale=None):
 
 
 class FlameletSolver:
-    """Plain drop-in: one flamelet, host loop per Newton iteration."""
+    """Plain drop-in: one flamelet, host loop per Newton iteration.
+
+    ``semantics`` (``FlameletEquations``): the default ``"reference"``
+    reproduces the reference's Newton iterates for mechanisms of at most 16
+    species (the register-Jacobian kernel); larger mechanisms use
+    ``"exact"``."""
 
     def __init__(self, domain, pyro_gas, mass_frac_bcs,
-                 newton_norm="reference"):
+                 newton_norm="reference", semantics=None):
         y_ox, y_fu = mass_frac_bcs
         self.domain = domain
-        self.gov_eqns = FlameletEquations(pyro_gas, domain, y_ox, y_fu)
+        if semantics is None:
+            semantics = ("reference" if pyro_gas._lib.has(
+                "pyro_flamelet_source") else "exact")
+        if semantics not in SEMANTICS:
+            raise ValueError(f"unknown semantics {semantics!r}")
+        self.semantics = semantics
+        self.gov_eqns = FlameletEquations(pyro_gas, domain, y_ox, y_fu,
+                                          semantics)
         self.num_v = pyro_gas.num_species + 1
         if newton_norm not in NEWTON_NORMS:
             raise ValueError(f"unknown Newton norm {newton_norm!r}")
         self.newton_norm = newton_norm
         self.log = []
+        self.failure = None
 
     # one Newton update v = -J^-1 R (solver.py:63-101)
     def _update(self, state, diss_rate, viscous_diss, temp_guess, pressure,
@@ -202,6 +229,11 @@ class FlameletSolver:
             if verbosity:
                 print(f"Newton iter {it}: time: {(time.time() - t0):.4e} s, "
                       f"|v| = {delta:.4e}")
+            if not np.isfinite(delta):
+                # a singular block or a temperature that did not converge:
+                # the reference would iterate on NaN; stop instead
+                self.failure = "non-finite update"
+                return state, it, delta, False
             if delta < tol:
                 return state, it, delta, True
             if err_prev < delta:
@@ -220,6 +252,9 @@ class FlameletSolver:
                                     time_step=time_step)
             state = state + v
             it += 1
+            if not np.isfinite(delta):
+                self.failure = "non-finite update"
+                break
         return state, it, delta
 
     def flamelet_time_march(self, verbosity, newton_maxiter, newton_tol,
@@ -232,10 +267,12 @@ class FlameletSolver:
             state, it, err = self.flamelet_time_step(
                 state, newton_maxiter, newton_tol, state_prev, time_step,
                 diss_rate, viscous_diss, temp, pressure, h_ox, h_fu)
+            if self.failure is not None:
+                break
             temp = self.gov_eqns.pyro_gas.get_temperature_from_enthalpy(
                 state.enthalpy, state.mass_fractions,
                 _t(temp_guess, state.enthalpy.device, state.enthalpy.numel()))
-            self.log.append(("cn", step, err))
+            self.log.append(("cn", step, err, float(temp.max())))
             if verbosity:
                 print(f"BDF time step {step}: time: {time.time() - t0:.4e} s, "
                       f"T_max = {float(temp.max()):.3f} K ")
@@ -256,10 +293,13 @@ class FlameletSolver:
         chi0[0], chi0[-1] = e0, e1
         vd = torch.zeros(n, dtype=torch.float64, device=dev)
         dt = dt0
+        self.failure = None
         for _ in range(steps):
             state, it, err = self.flamelet_time_step(
                 state, newton_maxiter, newton_tol, state, dt, chi0, vd,
                 temp_guess, pressure, h_ox, h_fu)
+            if self.failure is not None:
+                raise RuntimeError("equilibration failed: " + self.failure)
             dt *= growth
         temp = self.gov_eqns.pyro_gas.get_temperature_from_enthalpy(
             state.enthalpy, state.mass_fractions, _t(temp_guess, dev, n))
@@ -282,6 +322,7 @@ class FlameletSolver:
         dev = state_guess.enthalpy.device
         n = state_guess.enthalpy.numel()
         self.converged = False
+        self.failure = None
         if try_newton:
             state, it, err, ok = self._newton_loop_py(
                 verbosity, state_guess, newton_maxiter, newton_tol, diss_rate,
@@ -292,6 +333,9 @@ class FlameletSolver:
                     state.enthalpy, state.mass_fractions,
                     _t(temp_guess, dev, n))
         state, temp = state_guess, temp_guess
+        # a Newton try that failed (even on a non-finite update) falls back
+        # to the time march from the guess, as in the reference
+        self.failure = None
         for i in range(max_attempts):
             if verbosity:
                 print(f"Attempt {i} of BDF time marching")
@@ -299,9 +343,13 @@ class FlameletSolver:
                 verbosity, bdf_newton_maxiter, bdf_newton_tol, bdf_maxsteps,
                 bdf_time_step, diss_rate, viscous_diss, temp, pressure, h_ox,
                 h_fu, state)
+            if self.failure is not None:
+                break
             state_t, it, err, ok = self._newton_loop_py(
                 verbosity, state, newton_maxiter, newton_tol, diss_rate,
                 viscous_diss, temp, pressure, h_ox, h_fu)
+            if self.failure is not None:
+                break
             temp_t = gas.get_temperature_from_enthalpy(
                 state_t.enthalpy, state_t.mass_fractions,
                 _t(temp_guess, dev, n))
@@ -343,6 +391,8 @@ class FlameletResult:
     time_steps: int
     attempts: int
     finished_at: float = 0.0
+    #: why the solve failed (None when it converged)
+    failure: Optional[str] = None
 
 
 def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],

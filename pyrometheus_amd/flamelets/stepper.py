@@ -50,7 +50,14 @@ class SolveSettings:
 
 # per-flamelet integer/float control state, in this order
 _CTRL = ("phase", "it", "err_prev", "step", "inner", "attempt", "n_newton",
-         "n_steps", "eq_step")
+         "n_steps", "eq_step", "fail")
+#: ``fail`` codes: why a flamelet ended in DONE_FAIL
+FAIL_NONE, FAIL_ITER, FAIL_NONFINITE = 0, 1, 2
+FAIL_REASONS = {
+    FAIL_ITER: "no attempt converged (max_attempts reached)",
+    FAIL_NONFINITE: "update not finite: a singular block or a temperature "
+                    "that did not converge",
+}
 # per-flamelet device arrays: (nv, N) state-shaped and (N,) point-shaped
 _STATE = ("phi", "phi_guess", "phi_prev", "r_prev", "phi_save")
 _POINT = ("tg", "chi", "chi0", "vd", "pres")
@@ -147,7 +154,7 @@ class FlameletStepper:
             }
             ctrl = dict(phase=EQUIL if s.equilibrate_steps > 0 else first,
                         it=0, err_prev=np.inf, step=0, inner=0, attempt=0,
-                        n_newton=0, n_steps=0, eq_step=0)
+                        n_newton=0, n_steps=0, eq_step=0, fail=FAIL_NONE)
             slot = _Slot(self._next, p, ctrl, arrays, time.time())
             self._next += 1
             self._pending.append(slot)
@@ -332,10 +339,22 @@ class FlameletStepper:
         self.point_iterations += b.P
         now = time.time()
         it, err_prev = C["it"], C["err_prev"]
+        # a non-finite update cannot recover (a singular block, or a point
+        # whose temperature did not converge): stop that flamelet now -- but
+        # a failed Newton try still falls back to the time march from the
+        # guess, as any failed try does
+        bad_try = live & ~np.isfinite(delta) & (phase == NEWTON_TRY)
+        nonfinite = live & ~np.isfinite(delta) & ~bad_try
+        if nonfinite.any():
+            phase[nonfinite] = DONE_FAIL
+            C["fail"][nonfinite] = FAIL_NONFINITE
+            live = live & ~nonfinite
+            eq, cn = eq & ~nonfinite, cn & ~nonfinite
         newton = live & ((phase == NEWTON_TRY) | (phase == NEWTON))
         C["n_newton"] += newton
         ok = newton & (delta < s.newton_tol)
-        bad = newton & ~ok & ((err_prev < delta) | (it + 1 >= s.newton_maxiter))
+        bad = newton & ~ok & ((err_prev < delta) | (it + 1 >= s.newton_maxiter)
+                              | bad_try)
         cont = newton & ~ok & ~bad
         phase[ok] = DONE_OK
         err_prev[cont] = delta[cont]
@@ -353,6 +372,7 @@ class FlameletStepper:
             again = f1 & (C["attempt"] < s.max_attempts)
             phase[again], C["step"][again], C["inner"][again] = MARCH, 0, 0
             phase[f1 & ~again] = DONE_FAIL
+            C["fail"][f1 & ~again] = FAIL_ITER
         march = live & cn
         C["inner"][march] += 1
         step_done = march & ((delta <= s.bdf_newton_tol)
@@ -378,7 +398,9 @@ class FlameletStepper:
         if self.verbosity:
             print(f"batch iteration: live {int(live.sum())}, "
                   f"max |v| {np.max(delta[live]):.3e}")
-        done = live & ((phase == DONE_OK) | (phase == DONE_FAIL))
+        done = (live | nonfinite) & ((phase == DONE_OK)
+                                     | (phase == DONE_FAIL))
+        live = live | nonfinite
         if not done.any():
             return []
         # final temperature of the finished flamelets from their last guess
@@ -392,7 +414,8 @@ class FlameletStepper:
             out.append((sl.handle, FlameletResult(
                 st, b.T[a:e].clone(), bool(phase[j] == DONE_OK),
                 int(C["n_newton"][j]), int(C["n_steps"][j]),
-                int(C["attempt"][j]), now - sl.t_added)))
+                int(C["attempt"][j]), now - sl.t_added,
+                FAIL_REASONS.get(int(C["fail"][j])))))
         if done.sum() == live.sum():
             self._slots, self.batch = [], None
         else:

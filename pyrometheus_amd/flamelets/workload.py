@@ -93,10 +93,12 @@ def flamelet_batch(gas, n_flamelets=256, seed=4, n_min=64, n_max=1024,
 
 
 def solve_plain(gas, domain: Domain, prob: FlameletProblem,
-                equilibrate_steps=24, newton_norm="reference", **solve_kw):
+                equilibrate_steps=24, newton_norm="reference",
+                semantics="exact", **solve_kw):
     """The plain drop-in: one ``FlameletSolver`` per flamelet, equilibrium
     start, then the reference's hybrid ``solve``; returns
-    ``(state, temperature, converged)``."""
+    ``(state, temperature, converged)``.  ``semantics`` defaults to the
+    batched solver's (``"exact"``), so the two run the same iteration."""
     from .solver import FlameletSolver
     from .state import FlameletState
     kw = dict(verbosity=False, newton_maxiter=10, newton_tol=1e-8,
@@ -105,7 +107,7 @@ def solve_plain(gas, domain: Domain, prob: FlameletProblem,
     kw.update(solve_kw)
     dev = gas.device
     solver = FlameletSolver(domain, gas, (prob.y_ox, prob.y_fu),
-                            newton_norm=newton_norm)
+                            newton_norm=newton_norm, semantics=semantics)
     phi = torch.as_tensor(prob.phi, dtype=torch.float64, device=dev)
     state = FlameletState(phi[0].clone(), phi[1:].clone())
     tg = torch.as_tensor(prob.t_guess, dtype=torch.float64, device=dev)

@@ -180,7 +180,7 @@ MECH_SIGNATURES = {
                              _I64, _P, _I64, _P),
     "pyro_source_jacobian_work_doubles": (),
     "pyro_flamelet_source": (_P, _I64, _P, _P, _I64, _P, _I64, _P, _I64, _P,
-                             _P, _I64, _D, _I, _P, _P),
+                             _P, _I64, _D, _I, _P, _I, _P),
     "pyro_flamelet_assemble": (_P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P,
                                _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                _I64, _P),
@@ -190,6 +190,10 @@ MECH_SIGNATURES = {
                                          _P, _P, _P, _P, _P),
     "pyro_flamelet_update": (_P, _I64, _P, _P, _P, _P, _P, _P, _I64, _P,
                              _I64, _P),
+    "pyro_flamelet_eos_rt": (_P, _I64, _P, _I64, _P, _P, _I64, _I64, _D, _I,
+                             _P, _I, _P),
+    "pyro_flamelet_adjoint_blocks": (_P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                     _I64, _P),
 }
 
 

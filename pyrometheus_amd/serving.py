@@ -73,10 +73,19 @@ class Ticket:
     result: object = None
     _event: object = None
     _request: object = None
+    _done_event: object = None
 
     @property
     def latency(self) -> float:
         return self.t_done - self.t_arrival
+
+    @property
+    def event(self):
+        """Event recorded on the server's stream after the result was
+        written: wait on it (``stream.wait_event``) before reading the
+        result on another stream.  The ticket is finished once it has
+        completed."""
+        return self._done_event
 
 
 class ChemistryServer:
@@ -100,7 +109,7 @@ class ChemistryServer:
         self.streams_f: Dict[str, torch.cuda.Stream] = {}
         self.steppers: Dict[str, FlameletStepper] = {}
         self.queue: collections.deque = collections.deque()
-        self.wdot_inflight: List[Ticket] = []
+        self.inflight: List[Ticket] = []
         self.flamelets: Dict[Tuple[str, int], Ticket] = {}
         self.used = 0
         self.peak_used = 0
@@ -146,7 +155,9 @@ class ChemistryServer:
 
     @property
     def busy(self) -> bool:
-        return bool(self.queue or self.wdot_inflight or self.flamelets)
+        # inflight: production-rate requests and finished flamelets whose
+        # results are still being written on the device
+        return bool(self.queue or self.inflight or self.flamelets)
 
     def _admit(self):
         while self.queue:
@@ -168,7 +179,8 @@ class ChemistryServer:
                         req.rho, req.temperature, req.mass_fractions)
                     t._event = torch.cuda.Event()
                     t._event.record(self.stream_w)
-                self.wdot_inflight.append(t)
+                t._done_event = t._event
+                self.inflight.append(t)
             else:
                 st = self.steppers.get(t.mechanism)
                 if st is None:
@@ -187,13 +199,13 @@ class ChemistryServer:
 
     def poll(self):
         keep = []
-        for t in self.wdot_inflight:
+        for t in self.inflight:
             if t._event.query():
                 t._event = None
                 self._finish(t)
             else:
                 keep.append(t)
-        self.wdot_inflight = keep
+        self.inflight = keep
 
     def _stream_f(self, mech: str) -> torch.cuda.Stream:
         st = self.streams_f.get(mech)
@@ -216,12 +228,19 @@ class ChemistryServer:
                     launched.append((mech, st))
         self.poll()
         for mech, st in launched:
-            with torch.cuda.stream(self._stream_f(mech)):
+            stream = self._stream_f(mech)
+            with torch.cuda.stream(stream):
                 finished = st.complete()
             for h, res in finished:
                 t = self.flamelets.pop((mech, h))
                 t.result = res
-                self._finish(t)
+                # the result's final source evaluation and copies are still
+                # queued on the mechanism's stream: the ticket finishes when
+                # this event has completed
+                t._event = torch.cuda.Event()
+                t._event.record(stream)
+                t._done_event = t._event
+                self.inflight.append(t)
             self.poll()
         self.poll()
 

@@ -332,6 +332,47 @@ class ThermochemistryBase:
             raise RuntimeError("Temperature iteration failed to converge")
         return t
 
+    def get_temperature_jump_bands(self, mass_fractions, do_energy=True):
+        """Where ``get_temperature`` has no unique answer.
+
+        The NASA-7 fits of a mechanism need not make h(T) continuous at a
+        species' T_mid (the high set applies for ``T > T_mid``,
+        ``chem_expr.py:51-57``).  For GRI-3.0 every mixture's e(T) drops by
+        2e-5..5e-4 K c_v at T_mid = 1000 K, so targets there have two roots
+        (one per side) and the Newton iteration, here as in the reference,
+        returns whichever its guess leads to; where e(T) rises (GRI's HCNO at
+        1382 K) targets inside the jump have no root and the iteration does
+        not converge (``RuntimeError``, as in ``codegen/python.py:396``).
+
+        Returns ``(t_mid, jump, half_width)``: the distinct T_mid values (G,),
+        the jump e(T_mid+) - e(T_mid) (or of h) per cell in J/kg (G, *cells),
+        and |jump| / c_v(T_mid) (c_p for h) in K: a target from a temperature
+        within ``half_width`` of ``t_mid`` may have another root or none."""
+        y, cs = self._cells(mass_fractions)
+        mech = self._mech
+        tm = np.array([s.t_mid for s in mech.species])
+        groups = np.unique(tm)
+
+        def h_rt(c, t):
+            return (c[0] + c[1] * t / 2 + c[2] * t * t / 3 + c[3] * t ** 3 / 4
+                    + c[4] * t ** 4 / 5 + c[5] / t)
+        dmat = np.zeros((len(groups), mech.n_species))
+        for k, s in enumerate(mech.species):
+            g = int(np.searchsorted(groups, s.t_mid))
+            dmat[g, k] = (GAS_CONSTANT * s.t_mid
+                          * (h_rt(s.high, s.t_mid) - h_rt(s.low, s.t_mid))
+                          / mech.molecular_weights[k])
+        d = torch.tensor(dmat, dtype=torch.float64, device=self.device)
+        jump = d @ y
+        width = torch.empty_like(jump)
+        which = "cv" if do_energy is not False else "cp"
+        for g, t in enumerate(groups):
+            pv = self._mixture_thermo(which, float(t), y)
+            width[g] = jump[g].abs() / pv
+        t_mid = torch.tensor(groups, dtype=torch.float64, device=self.device)
+        return (t_mid, jump.reshape((len(groups),) + tuple(cs)),
+                width.reshape((len(groups),) + tuple(cs)))
+
     def get_temperature_from_enthalpy(self, enthalpy, mass_fractions,
                                       temp_init, tol=1.0e-10, maxiter=500):
         """Flamelet hook (``flamelets/make_pyro.py:255-299``): Newton on

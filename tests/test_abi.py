@@ -17,7 +17,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
-    return sorted(set(re.findall(r"^int\s+(pyro_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:int|int64_t)\s+(pyro_\w+)\s*\(",
+                                 text, re.M)))
 
 
 def test_registry_and_options():
@@ -57,3 +58,21 @@ def test_core_library_exports():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert set(names) <= set(runtime.CORE_SIGNATURES)
+
+
+def test_cli(tmp_path):
+    """``python -m pyrometheus_amd`` takes the reference's options
+    (pyrometheus/cli.py:11-42) and writes the hip target's source; the
+    language choices come from the registry and every option but the phase
+    is required."""
+    from pyrometheus_amd.cli import main
+    mech = os.path.join(ROOT, "pyrometheus_amd", "mechs", "sandiego.yaml")
+    out = tmp_path / "sd.hip"
+    assert main(["-l", "hip", "-m", mech, "-o", str(out), "-n",
+                 "sandiego"]) == 0
+    src = out.read_text()
+    assert src == HipCodeGenerator.generate("sandiego", bundled("sandiego"))
+    with pytest.raises(SystemExit):
+        main(["-l", "python", "-m", mech, "-o", str(out), "-n", "x"])
+    with pytest.raises(SystemExit):
+        main(["-l", "hip", "-m", mech, "-o", str(out)])

@@ -228,6 +228,18 @@ def test_autograd_function(device):
         tangent = fwAD.unpack_dual(out).tangent
     _, dw = wdot_jvp(gas, d(rho), d(T), d(Y), vr, vt, vy, want_primal=False)
     assert torch.equal(tangent, dw)
+    # a scalar (0-d) density and a (1,) temperature that require grad get
+    # the per-cell gradients summed over the cells they were broadcast to
+    r0 = torch.tensor(float(rho[0]), dtype=torch.float64, device=device,
+                      requires_grad=True)
+    t0 = torch.full((1,), float(T[0]), dtype=torch.float64, device=device,
+                    requires_grad=True)
+    (net_production_rates(gas, r0, t0, d(Y)) * wb).sum().backward()
+    g_r0, g_t0, _ = wdot_vjp(gas, r0.detach(), t0.detach(), d(Y), wb)
+    assert r0.grad.shape == () and t0.grad.shape == (1,)
+    assert torch.allclose(r0.grad, g_r0.sum(), rtol=1e-13, atol=0.0)
+    assert torch.allclose(t0.grad, g_t0.sum().reshape(1), rtol=1e-13,
+                          atol=0.0)
 
 
 @pytest.mark.gpu

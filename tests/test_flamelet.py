@@ -131,8 +131,12 @@ def test_gpu_source_jacobian_residual(gas, device, cluster):
                ONE_ATM, h_ox, h_fu)
     Rg = np.vstack([R.enthalpy.cpu().numpy()[None], R.mass_fractions.cpu().numpy()])
     # rounding scale of the stencil: the terms, not their (cancelling) sum
-    abc_abs = tuple(np.abs(c) for c in ref.abc)
-    lap = 0.5 * chi * F.apply_laplacian(abc_abs, np.abs(phi))
+    a, b, c = (np.abs(v) for v in ref.abc)
+    ap = np.abs(phi)
+    lap = b * ap
+    lap[:, 1:] += a[1:] * ap[:, :-1]
+    lap[:, :-1] += c[:-1] * ap[:, 1:]
+    lap *= 0.5 * chi
     rscale = lap + np.abs(S_ref) + np.abs(ref.boundary_conditions(h_ox, h_fu))
     rscale[1:] += sscale
     assert np.max(np.abs(Rg - R_ref) / (rscale + 1e-300)) < 1e-12
@@ -146,12 +150,14 @@ def test_gpu_source_jacobian_residual(gas, device, cluster):
 
 
 @pytest.mark.gpu
-def test_gpu_newton_step_and_solve(gas, device):
+@pytest.mark.parametrize("semantics", ["exact", "reference"])
+def test_gpu_newton_step_and_solve(gas, device, semantics):
     from pyrometheus_amd.flamelets.solver import FlameletSolver
     m, x, y_ox, y_fu, h_ox, h_fu, z_st, phi0, chi, T0 = _setup(n=33)
     vd = np.zeros(len(x))
-    ref = F.FlameletSolver(m, x, y_ox, y_fu)
-    solver = FlameletSolver(_domain(x), gas, (y_ox, y_fu))
+    ref = F.FlameletSolver(m, x, y_ox, y_fu, semantics=semantics)
+    solver = FlameletSolver(_domain(x), gas, (y_ox, y_fu),
+                            semantics=semantics)
     st = _state(phi0, device)
     args = (_dev(chi, device), _dev(vd, device), _dev(T0, device), ONE_ATM,
             h_ox, h_fu)
@@ -376,3 +382,83 @@ def test_gpu_53_species_flamelets(device):
         rel_h = np.abs(Rn[0]).max() / gross[0].max()
         rel_y = np.abs(Rn[1:]).max() / gross[1:].max()
         assert max(rel_h, rel_y) <= 1e-10, (rel_h, rel_y)
+
+
+@pytest.mark.gpu
+def test_gpu_notebook_case(gas, device):
+    """The reference notebook's run (examples/flamelet/flamelet.ipynb cells
+    2-7: sandiego, N = 101, T_ox = 500 K, chi_st = 100 then a restart at
+    1000, HP-equilibrium start) on the device FlameletSolver with its
+    default (reference) semantics: every printed T_max of both time marches
+    to 3 decimals, every printed Newton |v| above the rounding floor to 5
+    digits, the last one (rounding) within a factor 3 and below the
+    tolerance, the same iteration counts; the converged states match the
+    oracle's to 1e-10 in Y and 1e-7 K."""
+    from pyrometheus_amd.flamelets.solver import FlameletSolver
+    from tests.test_flamelet_pins import (NB_TMAX_100, NB_TMAX_1000,
+                                          NB_V_100, NB_V_1000,
+                                          ROUNDING_FLOOR, notebook_case)
+    m, x, y_ox, y_fu, h_ox, h_fu, z_st, phi, T = notebook_case()
+    solver = FlameletSolver(_domain(x), gas, (y_ox, y_fu))
+    assert solver.semantics == "reference"
+    ref = F.FlameletSolver(m, x, y_ox, y_fu, semantics="reference")
+    st, Td = _state(phi, device), _dev(T, device)
+    vd = np.zeros(len(x))
+    for chi_st, tmax, vn in ((100.0, NB_TMAX_100, NB_V_100),
+                             (1000.0, NB_TMAX_1000, NB_V_1000)):
+        print(f"[notebook] chi_st={chi_st}", flush=True)
+        chi = F.dissipation_profile(x, z_st, chi_st)
+        solver.log = []
+        st, Td = solver.solve(False, 10, 1e-8, 20, 1e-9, 1e-5, 10, False, 10,
+                              _dev(chi, device), _dev(vd, device), Td,
+                              ONE_ATM, h_ox, h_fu, st)
+        assert solver.converged
+        cn = [e for e in solver.log if e[0] == "cn"]
+        nw = [e for e in solver.log if e[0] == "newton"]
+        assert [f"{e[3]:.3f}" for e in cn] == tmax
+        assert len(nw) == len(vn)
+        for (_, it, v), r in zip(nw, vn):
+            if r > ROUNDING_FLOOR:
+                assert f"{v:.4e}" == f"{r:.4e}", (chi_st, it, v, r)
+            else:
+                assert v < 1e-8 and r / 3 <= v <= 3 * r, (chi_st, it, v, r)
+        phi, T, ok = ref.solve(10, 1e-8, 20, 1e-9, 1e-5, 10, False, 10, chi,
+                               vd, T, ONE_ATM, h_ox, h_fu, phi)
+        assert ok
+        Y = st.mass_fractions.cpu().numpy()
+        assert np.max(np.abs(Y - phi[1:])) <= 1e-10
+        assert np.max(np.abs(Td.cpu().numpy() - T)) <= 1e-7
+
+
+@pytest.mark.gpu
+def test_gpu_failed_flamelet_stops_at_once(gas, device):
+    """One flamelet of a batch starts with a NaN enthalpy at one point: that
+    point's temperature Newton does not converge, its source is NaN, the
+    flamelet's update is not finite and it ends failed after its first
+    iteration (not after every attempt), with the reason given; the other
+    flamelets of the batch give their results alone bit for bit.  The
+    plain solver stops on the same flamelet at once too."""
+    import copy
+    from pyrometheus_amd.flamelets.solver import FlameletSolver, solve_batch
+    probs = _problems(4, seed=4)
+    bad = copy.copy(probs[1])
+    bad.phi = probs[1].phi.copy()
+    bad.phi[0, len(bad.x) // 2] = np.nan
+    mix = solve_batch(gas, [probs[0], bad, probs[2], probs[3]],
+                      equilibrate_steps=24)
+    r = mix[1]
+    assert not r.converged and r.failure and "not finite" in r.failure
+    assert r.newton_iterations == 0 and r.time_steps == 0
+    for j in (0, 2, 3):
+        alone = solve_batch(gas, [probs[j]], equilibrate_steps=24)[0]
+        assert mix[j].converged and mix[j].failure is None
+        assert _same(alone.state.mass_fractions, mix[j].state.mass_fractions)
+        assert _same(alone.temperature, mix[j].temperature)
+    solver = FlameletSolver(_domain(bad.x), gas, (bad.y_ox, bad.y_fu))
+    st, _ = solver.solve(False, 10, 1e-8, 20, 1e-9, 1e-5, 10, False, 10,
+                         _dev(bad.chi, device),
+                         _dev(np.zeros(len(bad.x)), device),
+                         _dev(bad.t_guess, device), ONE_ATM, bad.h_ox,
+                         bad.h_fu, _state(bad.phi, device))
+    assert not solver.converged and solver.failure == "non-finite update"
+    assert len(solver.log) <= 1

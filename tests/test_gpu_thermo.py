@@ -238,3 +238,92 @@ def test_large_batch_properties(device):
     # mass balance: relative to |W w| the cancellation can be larger than
     # machine precision where w itself cancels, so bound it loosely
     assert float(torch.quantile(mass[::64], 0.99)) < 1e-10
+
+
+def _device_states(m, n, seed, device):
+    """T ~ U[300, 2500] K, Y = 0.5 Dirichlet(0.5) + 0.5 e_N2, drawn on the
+    device from a seeded generator."""
+    g = torch.Generator(device=device).manual_seed(seed)
+    T = 300.0 + 2200.0 * torch.rand(n, generator=g, dtype=torch.float64,
+                                    device=device)
+    alpha = torch.full((m.n_species, n), 0.5, dtype=torch.float64,
+                       device=device)
+    Y = torch._standard_gamma(alpha, generator=g)
+    Y /= Y.sum(0, keepdim=True)
+    Y *= 0.5
+    Y[m.species_index("N2")] += 0.5
+    guess = (T + 600.0 * (torch.rand(n, generator=g, dtype=torch.float64,
+                                     device=device) - 0.5)).clamp(200.0, 3500.0)
+    return T, Y, guess
+
+
+def test_temperature_jump_bands_large(device):
+    """T from e on 2^22 GRI-3.0 cells (guesses T +- 300 K), plus 8,192 cells
+    placed inside the T_mid jump bands of their mixture.
+
+    GRI-3.0's NASA-7 fits are not h-continuous at T_mid: e(T) drops by
+    2e-5..5e-4 K c_v at 1000 K, so a target from a temperature within the
+    band has a second root on the other side of T_mid.  Outside every band
+    (twice the half-width): |dT| <= 1e-9 K against the sampled temperature.
+    Inside: the result is a root (|e(T) - e*| <= 1e-9 K c_v by the oracle's
+    e) and the same root as the oracle's Newton from the same guess."""
+    m, gas, ref = setup("gri30", device)
+    n, nb = 1 << 22, 8192
+    print(f"[bands] gri30 N={n} + {nb} in-band cells", flush=True)
+    T, Y, guess = _device_states(m, n + nb, 21, device)
+    t_mid, jump, width = gas.get_temperature_jump_bands(Y)
+    # the last nb cells: T within the half-width of T_mid = 1000 K (or of
+    # the group with the widest band of that cell's mixture)
+    gsel = width[:, n:].argmax(0)
+    g = torch.Generator(device=device).manual_seed(22)
+    u = 2.0 * torch.rand(nb, generator=g, dtype=torch.float64,
+                         device=device) - 1.0
+    wb = width[:, n:].gather(0, gsel[None])[0]
+    T[n:] = t_mid[gsel] + u * wb
+    guess[n:] = (T[n:] + 600.0 * (torch.rand(nb, generator=g,
+                                             dtype=torch.float64,
+                                             device=device) - 0.5))
+    e = gas.get_mixture_internal_energy_mass(T, Y)
+    t_gpu = gas.get_temperature(e, guess, Y, True)
+    near = ((T[None, :] - t_mid[:, None]).abs() <= 2.0 * width + 1e-12).any(0)
+    out_err = float((t_gpu - T)[~near].abs().max())
+    n_band = int(near.sum())
+    print(f"  out-of-band max |dT| {out_err:.2e} K, in-band cells {n_band}, "
+          f"max half-width {float(width.max()):.2e} K", flush=True)
+    assert out_err <= 1e-9
+    assert n_band >= nb
+    idx = torch.nonzero(near).reshape(-1).cpu().numpy()
+    Tb, Yb = host(T)[idx], host(Y)[:, idx]
+    eb, gb, tb = host(e)[idx], host(guess)[idx], host(t_gpu)[idx]
+    cv = ref.get_mixture_specific_heat_cv_mass(tb, Yb)
+    resid = np.abs(ref.get_mixture_internal_energy_mass(tb, Yb) - eb) / cv
+    assert resid.max() <= 1e-9
+    t_ref = ref.get_temperature(eb, gb, Yb, True)
+    assert np.max(np.abs(tb - t_ref)) <= 1e-9
+    other = np.abs(tb - Tb) > 1e-9
+    print(f"  in-band: {int(other.sum())} cells took the other root, "
+          f"root residual {resid.max():.2e} K, vs oracle "
+          f"{np.max(np.abs(tb - t_ref)):.2e} K", flush=True)
+    assert other.any()  # the second root is real and reached
+
+
+def test_temperature_no_root_raises(device):
+    """A target inside GRI-3.0's upward e(T) jump at HCNO's T_mid = 1382 K
+    has no root: the Newton iteration alternates sides with steps of the
+    jump's size and, as the reference's generated code does, the call raises
+    RuntimeError (codegen/python.py:396)."""
+    m, gas, ref = setup("gri30", device)
+    Y = np.zeros((m.n_species, 1))
+    Y[m.species_index("HCNO")] = 1.0
+    Yd = dev(Y, device)
+    t_mid, jump, width = gas.get_temperature_jump_bands(Yd)
+    g = int(torch.argmin((t_mid - 1382.0).abs()))
+    assert float(t_mid[g]) == 1382.0 and float(jump[g, 0]) > 0.0
+    assert float(width[g, 0]) > 2e-6  # steps stay above the 1e-6 K tolerance
+    e_mid = gas.get_mixture_internal_energy_mass(1382.0, Yd)
+    target = e_mid + 0.5 * jump[g]
+    with pytest.raises(RuntimeError):
+        gas.get_temperature(target, 1300.0, Yd, True)
+    # the same target from the oracle: also no convergence
+    with pytest.raises(RuntimeError):
+        ref.get_temperature(host(target), 1300.0, Y, True)
