@@ -2,7 +2,7 @@
  * pyro_core.h -- C ABI of libpyro_core.so (mechanism-independent kernels).
  * Conventions as in pyro_hip.h: device pointers owned by the caller, float64,
  * asynchronous on `stream`, status 0 ok / -1 invalid argument / -2 launch
- * error / -3 unsupported size.
+ * error / -3 unsupported size / -4 beyond a fixed-size workspace.
  */
 #ifndef PYRO_CORE_H
 #define PYRO_CORE_H
@@ -36,6 +36,31 @@ int pyro_block_thomas_batched(const double* L, const double* C,
                               const int64_t* offsets, int64_t nsys, int nv,
                               int diag, double* work, int* n_singular,
                               pyro_stream_t stream);
+
+/* Doubles in the fixed workspace of pyro_block_thomas_batched_noalloc
+ * (8 Mi doubles = 64 MiB per device). */
+int64_t pyro_block_thomas_scratch_doubles(void);
+
+/* pyro_block_thomas_batched without an allocation or a workspace argument.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The workspace is a fixed array inside
+ * the library, one per device, held until the process ends.
+ *   n_rows: P = offsets[nsys], the block rows of the batch; calls with
+ *           n_rows * nv * (nv + 1) > pyro_block_thomas_scratch_doubles()
+ *           return -4 and launch nothing.  A system whose rows reach past
+ *           n_rows is not solved: its X is NaN and it counts in
+ *           *n_singular.
+ * Every call on one device shares that device's workspace (it holds the
+ * last call's factors; nothing in it needs to start at zero), so calls
+ * that overlap on one device race.  Otherwise as pyro_block_thomas_batched:
+ * same kernels, same results bit for bit. */
+int pyro_block_thomas_batched_noalloc(const double* L, const double* C,
+                                      const double* U, const double* R,
+                                      double* X, const int64_t* offsets,
+                                      int64_t nsys, int64_t n_rows, int nv,
+                                      int diag, int* n_singular,
+                                      pyro_stream_t stream);
 
 #ifdef __cplusplus
 }

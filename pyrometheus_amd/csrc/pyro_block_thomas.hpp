@@ -143,13 +143,24 @@ __global__ void __launch_bounds__(64)
 k_block_thomas(const double* __restrict__ L, const double* __restrict__ C,
                const double* __restrict__ U, const double* __restrict__ R,
                double* __restrict__ X, const int64_t* __restrict__ offsets,
-               double* __restrict__ work, int* __restrict__ n_singular) {
+               double* __restrict__ work, int64_t work_rows,
+               int* __restrict__ n_singular) {
   constexpr int W = 2 * NV + 1;
   const int s = blockIdx.x;
   const int lane = threadIdx.x;
   const int64_t off = offsets[s];
   const int64_t n = offsets[s + 1] - off;
   if (n <= 0) return;
+  // rows beyond the workspace (a fixed-size workspace and offsets that
+  // exceed the row count the caller stated): no solve, X = NaN, counted
+  // as singular
+  if (off < 0 || offsets[s + 1] > work_rows) {
+    if (lane < NV)
+      for (int64_t i = 0; i < n; ++i)
+        X[(off + i) * NV + lane] = __builtin_nan("");
+    if (lane == 0 && n_singular) atomicAdd(n_singular, 1);
+    return;
+  }
   const bool active = lane < NV;
   bool singular = false;
 

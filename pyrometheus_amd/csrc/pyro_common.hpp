@@ -16,6 +16,7 @@ enum Status : int {
   kInvalidArgument = -1,
   kLaunchError = -2,
   kUnsupported = -3,
+  kTooLarge = -4,  // beyond a fixed-size workspace
 };
 
 constexpr double kGasConstant = 8314.46261815324;  // J/(kmol K)

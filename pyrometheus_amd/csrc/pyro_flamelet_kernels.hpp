@@ -401,8 +401,8 @@ PYRO_API int pyro_flamelet_thomas(const double* L, const double* C,
   if (nsys < 0 || !L || !C || !U || !R || !X || !off || !work)
     return pyro::kInvalidArgument;
   pyro_bt::k_block_thomas<pyro_fl::NV, true>
-      <<<static_cast<unsigned>(nsys), 64, 0, stream>>>(L, C, U, R, X, off,
-                                                       work, n_singular);
+      <<<static_cast<unsigned>(nsys), 64, 0, stream>>>(
+          L, C, U, R, X, off, work, INT64_MAX, n_singular);
   return pyro::launch_status();
 }
 

@@ -74,6 +74,44 @@ def block_thomas_batched(lower: torch.Tensor, central: torch.Tensor,
     return out
 
 
+def block_thomas_batched_noalloc(lower: torch.Tensor, central: torch.Tensor,
+                                 upper: torch.Tensor, rhs: torch.Tensor,
+                                 offsets: torch.Tensor, out: torch.Tensor,
+                                 diag: bool = False,
+                                 n_singular: Optional[torch.Tensor] = None
+                                 ) -> torch.Tensor:
+    """``block_thomas_batched`` through the allocation-free entry point: the
+    library's fixed per-device workspace replaces ``work``, so nothing is
+    allocated (safe under graph capture).  Every argument, ``out`` included,
+    must already be a contiguous device tensor (``offsets`` int64,
+    ``n_singular`` a zeroed int32 counter or None); the batch may hold at
+    most ``scratch_doubles() // (nv * (nv + 1))`` block rows.  Calls that
+    overlap on one device race on the shared workspace."""
+    lib = load_core_library()
+    P, nv = rhs.shape
+    for t in (lower, central, upper, rhs, out, offsets):
+        if not t.is_contiguous() or t.device.type != "cuda":
+            raise TypeError("arguments must be contiguous device tensors")
+    if offsets.dtype != torch.int64 or out.shape != rhs.shape:
+        raise TypeError("offsets must be int64 and out shaped like rhs")
+    cap = scratch_doubles() // (nv * (nv + 1))
+    if P > cap:
+        raise ValueError(f"{P} block rows of size {nv} exceed the fixed "
+                         f"workspace ({cap} rows)")
+    lib.call("pyro_block_thomas_batched_noalloc", lower.data_ptr(),
+             central.data_ptr(), upper.data_ptr(), rhs.data_ptr(),
+             out.data_ptr(), offsets.data_ptr(), offsets.numel() - 1, P, nv,
+             int(diag), None if n_singular is None else n_singular.data_ptr(),
+             _stream())
+    return out
+
+
+def scratch_doubles() -> int:
+    """Size in doubles of the allocation-free entry point's workspace."""
+    return int(load_core_library().functions[
+        "pyro_block_thomas_scratch_doubles"]())
+
+
 def block_thomas(lower: torch.Tensor, central: torch.Tensor,
                  upper: torch.Tensor, rhs: torch.Tensor) -> torch.Tensor:
     """Drop-in for the reference ``block_thomas`` on one system."""

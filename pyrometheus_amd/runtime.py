@@ -254,6 +254,9 @@ CORE_SIGNATURES = {
     "pyro_block_thomas_supported": (_I,),
     "pyro_block_thomas_batched": (_P, _P, _P, _P, _P, _P, _I64, _I, _I, _P,
                                   _P, _P),
+    "pyro_block_thomas_scratch_doubles": (),
+    "pyro_block_thomas_batched_noalloc": (_P, _P, _P, _P, _P, _P, _I64, _I64,
+                                          _I, _I, _P, _P),
 }
 
 
