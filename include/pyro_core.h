@@ -62,6 +62,35 @@ int pyro_block_thomas_batched_noalloc(const double* L, const double* C,
                                       int diag, int* n_singular,
                                       pyro_stream_t stream);
 
+/* The per-flamelet control flow of a batched flamelet solve on the device:
+ * the decision of FlameletSolver.solve's hybrid strategy (flamelets/
+ * solver.py:103-147 Newton loop, :387-445 time march, :476-600 restarts)
+ * after one batch update, from the update's per-flamelet sums of squares
+ * norm2 (nsys, 2) [enthalpy row, species rows], with the update norm
+ * sqrt(a + b) or, when settings[11] != 0, sqrt(a / h_scale^2 + b).
+ * settings: 12 doubles in HOST memory (newton_maxiter, newton_tol,
+ * bdf_newton_maxiter, bdf_newton_tol, bdf_time_step, bdf_maxsteps,
+ * try_newton, max_attempts, equilibrate_steps, equilibrate_dt0,
+ * equilibrate_growth, scaled_norm).  ctl: (9, nsys) int32 device array
+ * (phase 0 Newton try / 1 time march / 2 Newton / 3 converged / 4 failed /
+ * 5 equilibration, it, step, inner, attempt, n_newton, n_steps, eq_step,
+ * fail), err_prev (nsys,) doubles; updated in place.  Writes the next
+ * update's controls modes, active, dt (as pyro_flamelet_assemble and
+ * pyro_flamelet_update read them) and per-flamelet action codes for
+ * pyro_flamelet_state_copy. */
+int pyro_flamelet_control(const double* settings, int* ctl, double* err_prev,
+                          const double* norm2, const double* h_scale,
+                          int* modes, int* active, double* dt, int* action,
+                          int64_t nsys, pyro_stream_t stream);
+
+/* The state copies the control step asked for, per point of each system:
+ * action 1 phi = phi_guess, 2 phi = phi_save, 3 phi_guess = phi,
+ * 4 phi_save = phi (all (nv, ld), n points, sys: system of each point). */
+int pyro_flamelet_state_copy(double* phi, double* phi_guess, double* phi_save,
+                             int64_t ld, int nv, const int* sys,
+                             const int* action, int64_t n,
+                             pyro_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,6 +2,7 @@
 #include <mutex>
 
 #include "pyro_block_thomas.hpp"
+#include "pyro_flamelet_control.hpp"
 
 #define PYRO_API extern "C" __attribute__((visibility("default")))
 
@@ -111,4 +112,65 @@ PYRO_API int pyro_block_thomas_batched_noalloc(
   if (st != pyro::kOk) return st;
   return dispatch_bt(L, C, U, R, X, offsets, nsys, nv, diag, work, n_rows,
                      n_singular, stream);
+}
+
+// Flamelet control on the device (pyro_flamelet_control.hpp).  settings
+// (host memory, 12 doubles): newton_maxiter, newton_tol,
+// bdf_newton_maxiter, bdf_newton_tol, bdf_time_step, bdf_maxsteps,
+// try_newton, max_attempts, equilibrate_steps, equilibrate_dt0,
+// equilibrate_growth, scaled_norm.  ctl (device, 9 x nsys int32): phase,
+// it, step, inner, attempt, n_newton, n_steps, eq_step, fail.
+PYRO_API int pyro_flamelet_control(const double* settings, int* ctl,
+                                   double* err_prev, const double* norm2,
+                                   const double* h_scale, int* modes,
+                                   int* active, double* dt, int* action,
+                                   int64_t nsys, hipStream_t stream) {
+  if (nsys == 0) return pyro::kOk;
+  if (nsys < 0 || nsys > 0x7fffffff || !settings || !ctl || !err_prev ||
+      !norm2 || !modes || !active || !dt || !action ||
+      (settings[11] != 0.0 && !h_scale))
+    return pyro::kInvalidArgument;
+  pyro_ctl::Settings c;
+  c.newton_maxiter = static_cast<int>(settings[0]);
+  c.newton_tol = settings[1];
+  c.bdf_newton_maxiter = static_cast<int>(settings[2]);
+  c.bdf_newton_tol = settings[3];
+  c.bdf_time_step = settings[4];
+  c.bdf_maxsteps = static_cast<int>(settings[5]);
+  c.try_newton = static_cast<int>(settings[6]);
+  c.max_attempts = static_cast<int>(settings[7]);
+  c.equilibrate_steps = static_cast<int>(settings[8]);
+  c.equilibrate_dt0 = settings[9];
+  c.equilibrate_growth = settings[10];
+  c.scaled_norm = static_cast<int>(settings[11]);
+  pyro_ctl::State st;
+  st.phase = ctl;
+  st.it = ctl + nsys;
+  st.step = ctl + 2 * nsys;
+  st.inner = ctl + 3 * nsys;
+  st.attempt = ctl + 4 * nsys;
+  st.n_newton = ctl + 5 * nsys;
+  st.n_steps = ctl + 6 * nsys;
+  st.eq_step = ctl + 7 * nsys;
+  st.fail = ctl + 8 * nsys;
+  st.err_prev = err_prev;
+  pyro_ctl::k_flamelet_control<<<pyro::grid_for(nsys, 256), 256, 0,
+                                 stream>>>(c, st, norm2, h_scale, modes,
+                                           active, dt, action,
+                                           static_cast<int>(nsys));
+  return pyro::launch_status();
+}
+
+PYRO_API int pyro_flamelet_state_copy(double* phi, double* phi_guess,
+                                      double* phi_save, int64_t ld, int nv,
+                                      const int* sys, const int* action,
+                                      int64_t n, hipStream_t stream) {
+  if (n == 0) return pyro::kOk;
+  if (n < 0 || ld < n || nv <= 0 || !phi || !phi_guess || !phi_save ||
+      !sys || !action)
+    return pyro::kInvalidArgument;
+  pyro_ctl::k_flamelet_state_copy<<<pyro::grid_for(n * nv, 256), 256, 0,
+                                    stream>>>(phi, phi_guess, phi_save, ld,
+                                              nv, sys, action, n);
+  return pyro::launch_status();
 }

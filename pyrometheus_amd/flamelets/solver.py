@@ -404,7 +404,9 @@ def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
                 kernel_events: Optional[dict] = None,
                 stats: Optional[dict] = None,
                 jacobian: str = "auto",
-                graphs: Optional[bool] = None) -> List[FlameletResult]:
+                graphs: Optional[bool] = None,
+                device_control: Optional[bool] = None,
+                sync_every: int = 8) -> List[FlameletResult]:
     """Solve every problem with the control flow of ``FlameletSolver.solve``
     applied to each flamelet independently (preceded, when
     ``equilibrate_steps`` > 0, by ``FlameletSolver.equilibrate``); one launch
@@ -415,7 +417,10 @@ def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
     receives the iteration count and the sum of batch points over them;
     ``jacobian`` picks the source-Jacobian kernels (``FlameletBatch``);
     ``graphs`` replays each iteration's launches from a captured HIP graph
-    (default: on a GPU unless ``kernel_events`` is given)."""
+    (default: on a GPU unless ``kernel_events`` is given);
+    ``device_control`` (default: with ``graphs``) runs every flamelet's
+    control flow on the device and reads it back once per ``sync_every``
+    iterations (the same results bit for bit)."""
     from .stepper import FlameletStepper, SolveSettings
     settings = SolveSettings(
         newton_maxiter=newton_maxiter, newton_tol=newton_tol,
@@ -426,7 +431,9 @@ def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
         equilibrate_growth=equilibrate_growth, newton_norm=newton_norm,
         jacobian=jacobian)
     st = FlameletStepper(pyro_gas, settings, verbosity=verbosity,
-                         kernel_events=kernel_events, graphs=graphs)
+                         kernel_events=kernel_events, graphs=graphs,
+                         device_control=device_control,
+                         sync_every=sync_every)
     handles = st.add(problems)
     results = st.run()
     if stats is not None:

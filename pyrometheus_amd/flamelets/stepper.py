@@ -73,11 +73,39 @@ class _Slot:
     finished_at: float = 0.0
 
 
+#: rows of the device control state, (len(_DCTL), B) int32, in this order
+_DCTL = ("phase", "it", "step", "inner", "attempt", "n_newton", "n_steps",
+         "eq_step", "fail")
+
+
+def _eq_dt_table(s: SolveSettings, n: int) -> np.ndarray:
+    """Equilibration time steps dt0 growth^k for k < n, each the product of
+    k factors taken left to right (the device control kernel computes the
+    same products, so both give the same bits)."""
+    d = [float(s.equilibrate_dt0)]
+    for _ in range(n - 1):
+        d.append(d[-1] * float(s.equilibrate_growth))
+    return np.array(d)
+
+
 class FlameletStepper:
+    """Advances a batch of flamelets, one Newton update of every live one
+    per iteration (see the module docstring).
+
+    ``device_control`` (default: with ``graphs``): every flamelet's control
+    flow runs on the device after each update (``pyro_flamelet_control``),
+    so ``launch()`` queues ``sync_every`` iterations back to back and the
+    host reads the control state only once per ``sync_every`` iterations.
+    The iterates are those of the host control flow bit for bit; finished
+    flamelets are masked on the device at once and retired by the host at
+    the next sync."""
+
     def __init__(self, gas, settings: Optional[SolveSettings] = None,
                  verbosity=False, device=None,
                  kernel_events: Optional[dict] = None,
-                 graphs: Optional[bool] = None):
+                 graphs: Optional[bool] = None,
+                 device_control: Optional[bool] = None,
+                 sync_every: int = 8):
         from .solver import NEWTON_NORMS
         self.gas = gas
         self.s = settings or SolveSettings()
@@ -104,6 +132,11 @@ class FlameletStepper:
             raise ValueError("graph replay and per-kernel events exclude "
                              "each other")
         self.graphs = bool(graphs)
+        if device_control is None:
+            device_control = self.graphs
+        self.device_control = bool(device_control)
+        self.sync_every = max(1, int(sync_every))
+        self._dt_eq = _eq_dt_table(self.s, max(1, self.s.equilibrate_steps))
         self._graph = None
         self._inflight = None  # (event, live, eq, cn) between launch/complete
         self.graph_captures = 0
@@ -253,6 +286,34 @@ class FlameletStepper:
         self.sys = b.sys.long()
         for sl in self._slots:  # the packed arrays now own the state
             sl.arrays = {}
+        self._controls_uploaded = False
+        if self.device_control:
+            from ..runtime import load_core_library
+            self._core = load_core_library()
+            rows = np.stack([self.C[k] for k in _DCTL]).astype(np.int32)
+            self._dctl = torch.as_tensor(rows, device=self.dev).contiguous()
+            self._derr = torch.as_tensor(self.C["err_prev"],
+                                         dtype=torch.float64,
+                                         device=self.dev).contiguous()
+            self._action = torch.zeros(b.B, dtype=torch.int32,
+                                       device=self.dev)
+            self._hscale = torch.as_tensor(
+                np.broadcast_to(self.h_scale, (b.B,)).copy(),
+                dtype=torch.float64, device=self.dev)
+            self._dctl_pin = torch.zeros(tuple(rows.shape), dtype=torch.int32,
+                                         pin_memory=pinned)
+            self._derr_pin = torch.zeros(b.B, dtype=torch.float64,
+                                         pin_memory=pinned)
+            s = self.s
+            self._settings = np.array(
+                [s.newton_maxiter, s.newton_tol, s.bdf_newton_maxiter,
+                 s.bdf_newton_tol, s.bdf_time_step, s.bdf_maxsteps,
+                 int(bool(s.try_newton)), s.max_attempts,
+                 s.equilibrate_steps, s.equilibrate_dt0,
+                 s.equilibrate_growth, int(s.newton_norm == "scaled")],
+                dtype=np.float64)
+            # flamelets already returned to the caller
+            self._retired = np.zeros(b.B, dtype=bool)
 
     # -- one iteration ----------------------------------------------------
     def _pts(self, mask_sys):
@@ -276,28 +337,13 @@ class FlameletStepper:
             self._pack()
         if self.batch is None:
             return False
-        s, b, A, C = self.s, self.batch, self.A, self.C
-        phase = C["phase"]
+        if self.device_control:
+            return self._launch_device()
+        live, eq, cn = self._upload_controls()
+        s, b, A = self.s, self.batch, self.A
         phi, tg, chi, chi0, vd, pres = (A["phi"], A["tg"], A["chi"],
                                         A["chi0"], A["vd"], A["pres"])
-        live = (phase != DONE_OK) & (phase != DONE_FAIL)
-        eq = phase == EQUIL
-        cn = (phase == MARCH) | eq
-        dts = np.where(eq, s.equilibrate_dt0 * s.equilibrate_growth
-                       ** C["eq_step"], s.bdf_time_step)
-        # the per-flamelet controls travel in one upload: modes (bit 0
-        # Crank-Nicolson, bit 1 the equilibration dissipation rate, bit 2
-        # start of a time step), the active flags and the time steps (b.dt
-        # is a view of the upload)
-        # a flamelet starting a time step (bit 2) has the assembly kernel
-        # store phi_prev = phi and r_prev = R(phi) before it uses them
-        nb, host = b.B, self._ctl_np
-        starting = cn & (C["inner"] == 0)
-        host[:nb] = (cn.astype(np.int32) | (eq.astype(np.int32) << 1)
-                     | (starting.astype(np.int32) << 2))
-        host[nb:2 * nb] = live
-        host[2 * nb:4 * nb].view(np.float64)[:] = dts
-        self._ctl_dev.copy_(self._ctl_pin, non_blocking=True)
+        nb = b.B
         modes, active = self._ctl_dev[:nb], self._ctl_dev[nb:2 * nb]
         def update():
             b.newton_update(phi, chi, vd, tg, pres, modes, A["phi_prev"],
@@ -316,6 +362,117 @@ class FlameletStepper:
         self._inflight = (ev, live, eq, cn)
         return True
 
+    def _upload_controls(self):
+        """Launch controls of every flamelet from the host control state,
+        uploaded in one copy; returns the (live, eq, cn) masks."""
+        s, b, C = self.s, self.batch, self.C
+        phase = C["phase"]
+        live = (phase != DONE_OK) & (phase != DONE_FAIL)
+        eq = phase == EQUIL
+        cn = (phase == MARCH) | eq
+        k = np.minimum(C["eq_step"], len(self._dt_eq) - 1)
+        dts = np.where(eq, self._dt_eq[k], s.bdf_time_step)
+        # the per-flamelet controls travel in one upload: modes (bit 0
+        # Crank-Nicolson, bit 1 the equilibration dissipation rate, bit 2
+        # start of a time step), the active flags and the time steps (b.dt
+        # is a view of the upload)
+        # a flamelet starting a time step (bit 2) has the assembly kernel
+        # store phi_prev = phi and r_prev = R(phi) before it uses them
+        nb, host = b.B, self._ctl_np
+        starting = cn & (C["inner"] == 0)
+        host[:nb] = (cn.astype(np.int32) | (eq.astype(np.int32) << 1)
+                     | (starting.astype(np.int32) << 2))
+        host[nb:2 * nb] = live
+        host[2 * nb:4 * nb].view(np.float64)[:] = dts
+        self._ctl_dev.copy_(self._ctl_pin, non_blocking=True)
+        return live, eq, cn
+
+    def _launch_device(self) -> bool:
+        """``launch`` with the control flow on the device: ``sync_every``
+        iterations (update, control decision, state copies) replayed back
+        to back, then the control state copied to host memory."""
+        b, A = self.batch, self.A
+        nb = b.B
+        if not self._controls_uploaded:
+            self._upload_controls()
+            self._controls_uploaded = True
+        phi, tg, chi, chi0, vd, pres = (A["phi"], A["tg"], A["chi"],
+                                        A["chi0"], A["vd"], A["pres"])
+        modes, active = self._ctl_dev[:nb], self._ctl_dev[nb:2 * nb]
+        core, st = self._core, lambda: torch.cuda.current_stream(
+            self.dev).cuda_stream
+        def iteration():
+            b.newton_update(phi, chi, vd, tg, pres, modes, A["phi_prev"],
+                            A["r_prev"], apply=True, active=active,
+                            chi0=chi0, refresh_guess=True)
+            core.call("pyro_flamelet_control",
+                      self._settings.ctypes.data, self._dctl.data_ptr(),
+                      self._derr.data_ptr(), b.norm2.data_ptr(),
+                      self._hscale.data_ptr(), modes.data_ptr(),
+                      active.data_ptr(), b.dt.data_ptr(),
+                      self._action.data_ptr(), nb, st())
+            core.call("pyro_flamelet_state_copy", phi.data_ptr(),
+                      A["phi_guess"].data_ptr(), A["phi_save"].data_ptr(),
+                      phi.stride(0), self.nv, b.sys.data_ptr(),
+                      self._action.data_ptr(), b.P, st())
+        k = self.sync_every
+        if self.graphs:
+            if self._graph is None:
+                self._graph = self._capture(iteration)
+            for _ in range(k):
+                self._graph.replay()
+        else:
+            for _ in range(k):
+                iteration()
+        self._dctl_pin.copy_(self._dctl, non_blocking=True)
+        self._derr_pin.copy_(self._derr, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.dev))
+        self._inflight = (ev, k)
+        return True
+
+    def _complete_device(self):
+        """``complete`` with the control flow on the device: read the
+        control state back, return the flamelets that finished since the
+        last sync."""
+        from .solver import FlameletResult
+        from .state import _array_to_state
+        ev, k = self._inflight
+        self._inflight = None
+        ev.synchronize()
+        b, A, C = self.batch, self.A, self.C
+        rows = self._dctl_pin.numpy()
+        for j, name in enumerate(_DCTL):
+            C[name][:] = rows[j]
+        C["err_prev"][:] = self._derr_pin.numpy()
+        self.iterations += k
+        self.point_iterations += k * b.P
+        now = time.time()
+        phase = C["phase"]
+        done = ((phase == DONE_OK) | (phase == DONE_FAIL)) & ~self._retired
+        if not done.any():
+            return []
+        self._retired |= done
+        phi, tg, vd, pres = A["phi"], A["tg"], A["vd"], A["pres"]
+        b.source(phi, vd, tg, pres, jac=False)
+        out = []
+        for j in np.nonzero(done)[0]:
+            sl = self._slots[j]
+            a, e = int(self.off[j]), int(self.off[j + 1])
+            sl.finished_at = now
+            out.append((sl.handle, FlameletResult(
+                _array_to_state(phi[:, a:e].clone()), b.T[a:e].clone(),
+                bool(phase[j] == DONE_OK), int(C["n_newton"][j]),
+                int(C["n_steps"][j]), int(C["attempt"][j]),
+                now - sl.t_added, FAIL_REASONS.get(int(C["fail"][j])))))
+        if self._retired.all():
+            self._slots, self.batch = [], None
+        else:
+            sizes = np.diff(self.off)
+            if sizes[self._retired].sum() * 2 > sizes.sum():
+                self._pack()
+        return out
+
     def complete(self):
         """Second half of ``step``: wait for the launched update, run every
         flamelet's control flow on its norms, retire the finished ones;
@@ -324,6 +481,8 @@ class FlameletStepper:
         from .state import _array_to_state
         if self._inflight is None:
             return []
+        if self.device_control:
+            return self._complete_device()
         ev, live, eq, cn = self._inflight
         self._inflight = None
         if ev is not None:

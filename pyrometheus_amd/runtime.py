@@ -257,6 +257,8 @@ CORE_SIGNATURES = {
     "pyro_block_thomas_scratch_doubles": (),
     "pyro_block_thomas_batched_noalloc": (_P, _P, _P, _P, _P, _P, _I64, _I64,
                                           _I, _I, _P, _P),
+    "pyro_flamelet_control": (_P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P),
+    "pyro_flamelet_state_copy": (_P, _P, _P, _I64, _I, _P, _P, _I64, _P),
 }
 
 

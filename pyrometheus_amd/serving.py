@@ -93,13 +93,17 @@ class ChemistryServer:
 
     def __init__(self, gases: Dict[str, object], hbm_budget: int = 32 << 30,
                  max_flamelets: int = 1024,
-                 settings: Optional[SolveSettings] = None, device=None):
+                 settings: Optional[SolveSettings] = None, device=None,
+                 sync_every: int = 4):
         self.gases = dict(gases)
         any_gas = next(iter(self.gases.values()))
         self.dev = torch.device(device) if device is not None \
             else any_gas.device
         self.budget = int(hbm_budget)
         self.max_flamelets = int(max_flamelets)
+        # flamelet iterations queued per host sync of a batch (control flow
+        # on the device in between; admissions happen at syncs)
+        self.sync_every = int(sync_every)
         self.settings = settings or SolveSettings(equilibrate_steps=24,
                                                   newton_norm="scaled")
         # both streams at the default priority
@@ -185,7 +189,8 @@ class ChemistryServer:
                 st = self.steppers.get(t.mechanism)
                 if st is None:
                     st = FlameletStepper(self.gases[t.mechanism],
-                                         self.settings, device=self.dev)
+                                         self.settings, device=self.dev,
+                                         sync_every=self.sync_every)
                     self.steppers[t.mechanism] = st
                 with torch.cuda.stream(self._stream_f(t.mechanism)):
                     (h,) = st.add([req.problem])

@@ -236,7 +236,7 @@ def test_gpu_graph_replay_matches_eager(gas, device):
     out = {}
     for graphs in (False, True):
         st = FlameletStepper(gas, SolveSettings(equilibrate_steps=24),
-                             graphs=graphs)
+                             graphs=graphs, device_control=False)
         st.norm_log = []
         handles = st.add(probs)
         res = st.run()
@@ -462,3 +462,41 @@ def test_gpu_failed_flamelet_stops_at_once(gas, device):
                          bad.h_fu, _state(bad.phi, device))
     assert not solver.converged and solver.failure == "non-finite update"
     assert len(solver.log) <= 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("norm,sync", [("reference", 1), ("reference", 8),
+                                       ("scaled", 5)])
+def test_gpu_device_control_matches_host(gas, device, norm, sync):
+    """The control flow on the device (pyro_flamelet_control after every
+    update, the host reading it back once per ``sync`` iterations) gives
+    the host control flow's results bit for bit: states, temperatures,
+    convergence, Newton and time-step counts, attempts; a flamelet alone
+    equals itself in the batch."""
+    from pyrometheus_amd.flamelets.stepper import (FlameletStepper,
+                                                   SolveSettings)
+    probs = _problems(7, seed=5)
+    out = {}
+    for dc in (False, True):
+        st = FlameletStepper(gas, SolveSettings(equilibrate_steps=24,
+                                                newton_norm=norm),
+                             graphs=True, device_control=dc,
+                             sync_every=sync)
+        handles = st.add(probs)
+        res = st.run()
+        out[dc] = [res[h] for h in handles]
+    for j, (a, b) in enumerate(zip(out[False], out[True])):
+        assert (a.converged, a.newton_iterations, a.time_steps,
+                a.attempts) == (b.converged, b.newton_iterations,
+                                b.time_steps, b.attempts), j
+        for nm, u, v in (("h", a.state.enthalpy, b.state.enthalpy),
+                         ("Y", a.state.mass_fractions,
+                          b.state.mass_fractions),
+                         ("T", a.temperature, b.temperature)):
+            assert _same(u, v), (j, nm)
+    st = FlameletStepper(gas, SolveSettings(equilibrate_steps=24,
+                                            newton_norm=norm),
+                         graphs=True, device_control=True, sync_every=sync)
+    (h,) = st.add([probs[3]])
+    alone = st.run()[h]
+    assert _same(alone.state.mass_fractions, out[True][3].state.mass_fractions)
