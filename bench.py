@@ -20,6 +20,9 @@ production rates and the full (ns+1)^2 flamelet-source Jacobian on 2^22
 GRI-3.0 cells (seed 3, cotangent seed 33), with a central-difference check
 of the VJP at h = 1e-6 T.
 
+The ``transport`` object times the mixture-averaged transport properties
+(Wilke viscosity, conductivity, species diffusivities) on 2^22 GRI-3.0 cells.
+
 The ``flamelets`` object of the same line solves 256 steady flamelets of the
 9-species H2/air mechanism (grid sizes uniform in [64, 1024] clustered about
 Z_st, chi_st log-uniform in [1, 1000] 1/s, fuel H2:N2 at 300 K, air at 800 K,
@@ -79,6 +82,8 @@ def parse():
     ap.add_argument("--deriv-cells", type=int, default=1 << 22)
     ap.add_argument("--no-derivatives", action="store_true")
     ap.add_argument("--no-pointwise", action="store_true")
+    ap.add_argument("--transport-cells", type=int, default=1 << 22)
+    ap.add_argument("--no-transport", action="store_true")
     ap.add_argument("--no-wdot", action="store_true",
                     help="skip the omega-dot leg (profiling the flamelets)")
     ap.add_argument("--traffic-json",
@@ -583,6 +588,82 @@ def max_scaled_residual(gas, probs, results):
                           "|b phi| + |c phi_+|) + |S|)"}
 
 
+def transport_leg(args, device):
+    """Mixture-averaged transport on GRI-3.0: Wilke's viscosity, the
+    conductivity mean and the species diffusivities over 2^22 cells, each
+    timed with HIP events on the launch stream.  The double sums over
+    species pairs make the first and last compute-bound: the FP64 rate
+    counts the operations of the pair loop per cell (7 per pair for Wilke,
+    19 per pair for the diffusivities: the ln-T polynomial, the
+    reciprocal's estimate refinement and the accumulation).  The CPU line
+    times the numpy oracle on 4,096 of the cells, one core."""
+    import numpy as np
+    import torch
+    from oracle.transport_np import Transport
+    from pyrometheus_amd import bundled, get_thermochem_class
+    mech = bundled("gri30")
+    gas = get_thermochem_class(mech)(device=device)
+    n, ns = args.transport_cells, mech.n_species
+    T, p, Y = device_inputs(mech, n, args.seed + 5, device)
+    print("[bench] transport", file=sys.stderr, flush=True)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms.append(a.elapsed_time(b))
+        return sum(ms) / len(ms)
+
+    res = {"workload": f"gri30, {n} cells (seed {args.seed + 5}), "
+                       "fits of pyrometheus_amd.transport"}
+    legs = (("mixture_viscosity_wilke",
+             lambda: gas.get_mixture_viscosity_mixavg(T, Y), 7 * ns * ns,
+             8 * (ns + 2)),
+            ("mixture_conductivity",
+             lambda: gas.get_mixture_thermal_conductivity_mixavg(T, Y),
+             None, 8 * (ns + 2)),
+            ("species_diffusivities_mixavg",
+             lambda: gas.get_species_mass_diffusivities_mixavg(p, T, Y),
+             19 * ns * ns, 8 * (2 * ns + 2)))
+    for name, fn, flop, nbytes in legs:
+        ms = timed(fn)
+        e = {"ms": ms, "mcells_s": n / ms / 1e3, "bytes_per_cell": nbytes,
+             "hbm_frac": nbytes * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        if flop:
+            tf = flop * n / (ms * 1e-3) / 1e12
+            e.update(flop_per_cell=flop, fp64_tflops=tf,
+                     fp64_frac=tf / FP64_PEAK_TFLOPS)
+        res[name] = e
+    # CPU: the oracle on a slice, evaluated on the library's own fits
+    m = 4096
+    Tc, pc, Yc = (T[:m].cpu().numpy(), p[:m].cpu().numpy(),
+                  Y[:, :m].cpu().numpy())
+    ref = Transport(mech, *gas.get_transport_fits())
+    cpu = {}
+    for name, fn in (("mixture_viscosity_wilke",
+                      lambda: ref.get_mixture_viscosity_mixavg(Tc, Yc)),
+                     ("species_diffusivities_mixavg",
+                      lambda: ref.get_species_mass_diffusivities_mixavg(
+                          pc, Tc, Yc))):
+        t0 = time.perf_counter()
+        fn()
+        cpu[name] = m / (time.perf_counter() - t0) / 1e6
+    res["cpu_baseline"] = {"unit": "Mcells/s", "cores": 1, "kind": "port",
+                           "sample": f"{m} of the cells, numpy oracle",
+                           **cpu}
+    del T, p, Y
+    torch.cuda.empty_cache()
+    return res
+
+
 def config1_leg(args, device):
     """The CPU-runnable configuration on the device: net production rates
     and T from e (guesses T +- 300 K) of the 9-species synthetic H2-like
@@ -714,6 +795,8 @@ def main():
             line["pointwise"] = pointwise_leg(args, device)
         if not args.no_derivatives:
             line["derivatives"] = derivatives_leg(args, device)
+        if not args.no_transport:
+            line["transport"] = transport_leg(args, device)
         if not args.no_flamelets:
             _, line["flamelets"] = flamelet_leg(args, device)
         if not args.no_serving:
@@ -796,6 +879,8 @@ def main():
         line["pointwise"] = pointwise_leg(args, device)
     if not args.no_derivatives:
         line["derivatives"] = derivatives_leg(args, device)
+    if not args.no_transport:
+        line["transport"] = transport_leg(args, device)
     line["config1"] = config1_leg(args, device)
     cpu_probs = None
     if not args.no_flamelets:

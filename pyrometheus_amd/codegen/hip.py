@@ -47,6 +47,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 from ..mechanism import (GAS_CONSTANT, ONE_ATM, Mechanism, Reaction,
                          as_mechanism)
 from . import CodeGenerationOptions, CodeGenerator
+from .hip_transport import emit_transport_tables
 
 ABI_VERSION = 1
 #: mechanisms up to this many species also get the dense-Jacobian kernels
@@ -2018,12 +2019,15 @@ class HipCodeGenerator(CodeGenerator):
         dense = mech.n_species <= DENSE_JAC_MAX_SPECIES
         if dense:
             _emit_wdot_jac(w, L)
+        transport = emit_transport_tables(w, mech, lit)
         w("}  // namespace mech")
         w("")
         w('#include "pyro_thermo_kernels.hpp"')
         w('#include "pyro_flamelet_kernels.hpp"')
         if dense:
             w('#include "pyro_flamelet_dense.hpp"')
+        if transport:
+            w('#include "pyro_transport_kernels.hpp"')
         return w.text()
 
     @staticmethod

@@ -85,6 +85,10 @@ class Species:
     low: Tuple[float, ...]
     high: Tuple[float, ...]
     weight: float
+    #: gas transport data of the YAML ``transport`` entry: (geometry,
+    #: diameter [Angstrom], well-depth [K], dipole [Debye], polarizability
+    #: [Angstrom^3], rotational-relaxation) or None
+    transport: Optional[Tuple] = None
 
     def n_atoms(self, element: str) -> float:
         return dict(self.composition).get(element, 0.0)
@@ -394,11 +398,20 @@ def mechanism_from_dict(doc: dict, phase: Optional[str] = None,
             low = tuple(float(x) for x in data[0])
             high = tuple(float(x) for x in data[1])
         comp = {k: float(v) for k, v in s["composition"].items()}
+        tr_doc = s.get("transport")
+        transport = None
+        if tr_doc is not None and tr_doc.get("model", "gas") == "gas":
+            transport = (str(tr_doc["geometry"]),
+                         float(tr_doc["diameter"]),
+                         float(tr_doc["well-depth"]),
+                         float(tr_doc.get("dipole", 0.0)),
+                         float(tr_doc.get("polarizability", 0.0)),
+                         float(tr_doc.get("rotational-relaxation", 0.0)))
         species.append(Species(
             name=nm,
             composition=tuple(sorted(comp.items())),
             t_min=t_min, t_mid=t_mid, t_max=t_max, low=low, high=high,
-            weight=_species_weight(comp, elements)))
+            weight=_species_weight(comp, elements), transport=transport))
 
     rx_section = ph.get("reactions", "all")
     if rx_section in ("all", "declared-species"):
@@ -495,13 +508,20 @@ def to_dict(mech: Mechanism) -> dict:
 
     sp = []
     for s in mech.species:
-        sp.append({
+        d = {
             "name": s.name,
             "composition": dict(s.composition),
             "thermo": {"model": "NASA7",
                        "temperature-ranges": [s.t_min, s.t_mid, s.t_max],
                        "data": [list(s.low), list(s.high)]},
-        })
+        }
+        if s.transport is not None:
+            g, dia, eps, dip, pol, rot = s.transport
+            d["transport"] = {"model": "gas", "geometry": g,
+                              "diameter": dia, "well-depth": eps,
+                              "dipole": dip, "polarizability": pol,
+                              "rotational-relaxation": rot}
+        sp.append(d)
     rx = []
     for r in mech.reactions:
         d = {"equation": r.equation}

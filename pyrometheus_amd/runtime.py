@@ -75,8 +75,10 @@ def source_digest() -> str:
         for f in files:
             with open(os.path.join(CSRC_DIR, f), "rb") as fh:
                 h.update(fh.read())
-        with open(os.path.join(PKG_DIR, "codegen", "hip.py"), "rb") as fh:
-            h.update(fh.read())
+        for f in (("codegen", "hip.py"), ("codegen", "hip_transport.py"),
+                  ("transport.py",)):
+            with open(os.path.join(PKG_DIR, *f), "rb") as fh:
+                h.update(fh.read())
         _SRC_DIGEST = h.hexdigest()[:8]
     return _SRC_DIGEST
 
@@ -133,7 +135,8 @@ def prune_stale_libraries(keep) -> list:
     removed = []
     for f in sorted(os.listdir(GEN_DIR)):
         stem, ext = os.path.splitext(f)
-        if ext not in (".so", ".hip") or not stem.startswith("libpyro_"):
+        if ext not in (".so", ".hip", ".hpp") or \
+                not stem.startswith("libpyro_"):
             continue
         if stem in keep or stem.endswith("_" + src):
             continue
@@ -192,6 +195,10 @@ MECH_SIGNATURES = {
                              _I64, _P),
     "pyro_flamelet_eos_rt": (_P, _I64, _P, _I64, _P, _P, _I64, _I64, _D, _I,
                              _P, _I, _P),
+    "pyro_species_transport": (_I, _P, _I64, _P, _I64, _I64, _P),
+    "pyro_mixture_transport": (_I, _P, _I64, _P, _I64, _P, _I64, _P, _I64,
+                               _I64, _P),
+    "pyro_transport_fits": (_I, _P, _I64),
     "pyro_flamelet_adjoint_blocks": (_P, _P, _P, _P, _P, _P, _P, _P, _P,
                                      _I64, _P),
 }

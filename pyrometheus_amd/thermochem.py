@@ -28,7 +28,7 @@ import numpy as np
 import torch
 
 from .mechanism import GAS_CONSTANT, ONE_ATM, Mechanism
-from .runtime import HipLibrary, load_mechanism_library
+from .runtime import HipError, HipLibrary, load_mechanism_library
 
 _SPECIES_THERMO = {"cp_r": 0, "h_rt": 1, "s_r": 2, "g_rt": 3, "dcp_r": 4,
                    "dh_rt": 5, "ds_r": 6}
@@ -306,6 +306,84 @@ class ThermochemistryBase:
     def get_destruction_rates(self, rho, temperature, mass_fractions):
         return self.get_creation_destruction_rates(rho, temperature,
                                                    mass_fractions)[1]
+
+    # -- transport (codegen/python.py:564-641) ---------------------------------
+    def _transport_lib(self):
+        if not self._lib.has("pyro_species_transport"):
+            raise HipError(f"mechanism {self._mech.name!r} has no transport "
+                           "data (its library exports no transport kernels)")
+        return self._lib
+
+    def _species_transport(self, which, temperature, rows):
+        lib = self._transport_lib()
+        if isinstance(temperature, Number):
+            temperature = torch.tensor(float(temperature), dtype=torch.float64,
+                                       device=self.device)
+        t = temperature.reshape(-1).contiguous()
+        n = t.numel()
+        out = torch.empty(rows + (n,), dtype=torch.float64, device=self.device)
+        lib.call("pyro_species_transport", which, t.data_ptr(), 1,
+                 out.data_ptr(), n, n, _stream())
+        return out.reshape(rows + tuple(temperature.shape))
+
+    def get_species_viscosities(self, temperature):
+        """Species viscosities [Pa s], (ns, *cells)."""
+        return self._species_transport(0, temperature, (self.num_species,))
+
+    def get_species_thermal_conductivities(self, temperature):
+        """Species thermal conductivities [W/(m K)], (ns, *cells)."""
+        return self._species_transport(1, temperature, (self.num_species,))
+
+    def get_species_binary_mass_diffusivities(self, temperature):
+        """Binary diffusion coefficients times pressure [m^2/s Pa],
+        (ns, ns, *cells)."""
+        ns = self.num_species
+        return self._species_transport(2, temperature, (ns, ns))
+
+    def _mixture_transport(self, which, pressure, temperature,
+                           mass_fractions):
+        lib = self._transport_lib()
+        y, cs = self._cells(mass_fractions)
+        n = y.shape[1]
+        t, ts = self._scalar_field(temperature, n, cs)
+        p, ps = ((None, 0) if pressure is None
+                 else self._scalar_field(pressure, n, cs))
+        rows = self.num_species if which == 2 else None
+        out = self._out(rows, n)
+        lib.call("pyro_mixture_transport", which, _ptr(p), ps, t.data_ptr(),
+                 ts, y.data_ptr(), y.stride(0), out.data_ptr(), n, n,
+                 _stream())
+        return self._shape(out, rows, cs)
+
+    def get_mixture_viscosity_mixavg(self, temperature, mass_fractions):
+        """Mixture viscosity by Wilke's rule [Pa s]."""
+        return self._mixture_transport(0, None, temperature, mass_fractions)
+
+    def get_mixture_thermal_conductivity_mixavg(self, temperature,
+                                                mass_fractions):
+        """Mixture conductivity, the mean of the arithmetic and harmonic
+        mole-fraction averages [W/(m K)]."""
+        return self._mixture_transport(1, None, temperature, mass_fractions)
+
+    def get_species_mass_diffusivities_mixavg(self, pressure, temperature,
+                                              mass_fractions):
+        """Mixture-averaged species diffusivities [m^2/s], (ns, *cells)."""
+        return self._mixture_transport(2, pressure, temperature,
+                                       mass_fractions)
+
+    def get_transport_fits(self):
+        """The fit coefficients the library was built with: viscosity
+        (ns, 5), conductivity (ns, 5), binary diffusivity (ns, ns, 5), in
+        ascending powers of ln T (pyrometheus_amd.transport)."""
+        lib = self._transport_lib()
+        ns = self.num_species
+        out = []
+        for which, shape in ((0, (ns, 5)), (1, (ns, 5)), (2, (ns, ns, 5))):
+            a = np.zeros(shape)
+            lib.call("pyro_transport_fits", which,
+                     a.ctypes.data_as(ctypes.c_void_p), a.size)
+            out.append(a)
+        return tuple(out)
 
     # -- temperature (codegen/python.py:375-396; make_pyro.py:255-299) ---------
     def _temperature(self, target, t_guess, y, do_energy, mode, tol, maxit):
