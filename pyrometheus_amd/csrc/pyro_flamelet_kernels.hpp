@@ -12,7 +12,7 @@
 #pragma once
 
 #include "pyro_block_thomas.hpp"
-#include "pyro_spike.hpp"
+#include "pyro_spike_mfma.hpp"
 
 namespace pyro_fl {
 
@@ -452,6 +452,20 @@ PYRO_API int pyro_flamelet_thomas_partitioned(
     return pyro_spike::solve<pyro_fl::NV>(
         L, C, U, R, X, off, pstart, pend, psys, nparts, jbefore, jafter,
         roff, nsys, nj, fact, maps, red, Z, n_singular, stream);
+  }
+}
+
+// Sweep form of the partitioned solve for 16 < NV <= 63: 1 blocked
+// Gauss-Jordan with MFMA trailing updates (default), 0 the row-per-lane
+// eight-wave sweep; a negative form only queries.  Returns the previous
+// form, or -3 (unsupported) when the library's NV has one form.
+PYRO_API int pyro_flamelet_thomas_sweep_form(int form) {
+  if constexpr (pyro_fl::NV <= 16 || pyro_fl::NV > 63) {
+    return pyro::kUnsupported;
+  } else {
+    const int prev = pyro_spike::sweep_form();
+    if (form == 0 || form == 1) pyro_spike::sweep_form() = form;
+    return prev;
   }
 }
 

@@ -751,37 +751,4 @@ k_spike_reduced_waves(const double* __restrict__ maps,
   }
 }
 
-// Host side: the three launches.  All arrays are device arrays prepared by
-// the caller (see pyro_flamelet_thomas_partitioned).
-template <int NV>
-int solve(const double* L, const double* C, const double* U, const double* R,
-          double* X, const int64_t* off, const int64_t* pstart,
-          const int64_t* pend, const int* psys, int64_t nparts,
-          const int64_t* jbefore, const int64_t* jafter, const int64_t* roff,
-          int64_t nsys, int64_t nj, double* fact, double* maps, double* red,
-          double* Z, int* n_singular, hipStream_t stream) {
-  if constexpr (NV <= 16) {
-    k_spike_sweep_grp<NV>
-        <<<static_cast<unsigned>((2 * nparts + 3) / 4), 64, 0, stream>>>(
-            L, C, U, R, off, pstart, pend, psys, nparts, fact, maps,
-            n_singular);
-  } else {
-    k_spike_sweep_waves<NV>
-        <<<static_cast<unsigned>(2 * nparts), 64 * kWaves, 0, stream>>>(
-            L, C, U, R, off, pstart, pend, psys, fact, maps, n_singular);
-  }
-  if (nj > 0) {
-    if constexpr (NV <= 16)
-      k_spike_reduced<NV><<<static_cast<unsigned>(nsys), 64, 0, stream>>>(
-          maps, roff, red, Z, n_singular);
-    else
-      k_spike_reduced_waves<NV>
-          <<<static_cast<unsigned>(nsys), 64 * kWaves, 0, stream>>>(
-              maps, roff, red, Z, n_singular);
-  }
-  k_spike_recover<NV><<<static_cast<unsigned>(nparts), 64, 0, stream>>>(
-      fact, pstart, pend, jbefore, jafter, Z, X);
-  return pyro::launch_status();
-}
-
 }  // namespace pyro_spike

@@ -9,6 +9,11 @@ which alpha, depends on the block size, on the device and on the spread of
 flamelet sizes: short partitions mean short sweeps but a longer interface
 recursion.
 
+For 16 < nv <= 63 the partitioned sweep has two forms: ``"mfma"``, a
+blocked Gauss-Jordan whose trailing updates run on the FP64 matrix cores,
+and ``"rowlane"``, one logical row per lane with the pivot row broadcast by
+``v_readlane``; both are candidates.
+
 The schedule is kept per (device, nv), never per batch: a flamelet's
 partitioning then depends on its own size only, so its arithmetic -- and its
 result, bit for bit -- is the same alone and in any batch.  ``tune_thomas``
@@ -26,7 +31,8 @@ from typing import Dict, Optional, Sequence, Tuple
 import numpy as np
 import torch
 
-DEFAULT = {"kind": "partitioned", "alpha": math.sqrt(2.0)}
+DEFAULT = {"kind": "partitioned", "alpha": math.sqrt(2.0), "sweep": "mfma"}
+SWEEPS = {"rowlane": 0, "mfma": 1}
 ALPHAS = (0.5, 0.75, 1.0, math.sqrt(2.0), 2.0, 3.0)
 
 _SCHEDULES: Dict[Tuple[str, int], dict] = {}
@@ -46,11 +52,13 @@ def schedule(nv: int, device) -> dict:
 
 
 def set_schedule(nv: int, device, sched: dict):
-    if sched.get("kind") not in ("sequential", "partitioned"):
+    if sched.get("kind") not in ("sequential", "partitioned") or \
+            sched.get("sweep", DEFAULT["sweep"]) not in SWEEPS:
         raise ValueError(f"unknown block-Thomas schedule {sched!r}")
-    _SCHEDULES[_key(nv, device)] = {"kind": sched["kind"],
-                                    "alpha": float(sched.get("alpha",
-                                                             DEFAULT["alpha"]))}
+    _SCHEDULES[_key(nv, device)] = {
+        "kind": sched["kind"],
+        "alpha": float(sched.get("alpha", DEFAULT["alpha"])),
+        "sweep": sched.get("sweep", DEFAULT["sweep"])}
 
 
 def reset():
@@ -89,13 +97,15 @@ def tune_thomas(gas, sizes: Optional[Sequence[int]] = None,
     if sizes is None:
         sizes = np.random.default_rng(seed).integers(64, 1025, 256)
     b = _synthetic_batch(gas, sizes, seed, device)
-    candidates = [("sequential", None)]
+    candidates = [("sequential", None, DEFAULT["sweep"])]
     if b.supports_partitioned():
-        candidates += [("partitioned", float(a)) for a in alphas]
+        sweeps = ("mfma", "rowlane") if b.has_sweep_forms() else ("mfma",)
+        candidates += [("partitioned", float(a), sw) for a in alphas
+                       for sw in sweeps]
     stream = torch.cuda.current_stream(b.dev)
     times, ref = {}, None
-    for kind, alpha in candidates:
-        b.set_thomas(kind, alpha)
+    for kind, alpha, sweep in candidates:
+        b.set_thomas(kind, alpha, sweep=sweep)
         b.thomas()
         torch.cuda.synchronize(b.dev)
         x = b.x.clone()
@@ -105,7 +115,8 @@ def tune_thomas(gas, sizes: Optional[Sequence[int]] = None,
             err = float(((x - ref).abs().max() / ref.abs().max()).item())
             if not err <= 1e-8:
                 raise RuntimeError(f"block-Thomas schedule {kind} "
-                                   f"alpha={alpha} disagrees ({err:.2e})")
+                                   f"alpha={alpha} {sweep} disagrees "
+                                   f"({err:.2e})")
         ms = []
         for _ in range(reps):
             e0 = torch.cuda.Event(enable_timing=True)
@@ -116,10 +127,13 @@ def tune_thomas(gas, sizes: Optional[Sequence[int]] = None,
             torch.cuda.synchronize(b.dev)
             ms.append(e0.elapsed_time(e1))
         name = kind if alpha is None else f"{kind} alpha={alpha:.3g}"
-        times[name] = (float(np.median(ms)), kind, alpha)
+        if alpha is not None and b.has_sweep_forms():
+            name += f" {sweep}"
+        times[name] = (float(np.median(ms)), kind, alpha, sweep)
     best = min(times.values(), key=lambda t: t[0])
     chosen = {"kind": best[1],
-              "alpha": best[2] if best[2] is not None else DEFAULT["alpha"]}
+              "alpha": best[2] if best[2] is not None else DEFAULT["alpha"],
+              "sweep": best[3]}
     if store:
         set_schedule(b.nv, device, chosen)
     return dict(chosen, ms={k: v[0] for k, v in times.items()},

@@ -165,25 +165,36 @@ class FlameletBatch:
         # "partitioned" force one
         if thomas not in ("auto", "sequential", "partitioned"):
             raise ValueError(f"unknown block-Thomas schedule {thomas!r}")
-        alpha = None
+        alpha, sweep = None, "mfma"
         if thomas == "auto":
             from .autotune import schedule
             sch = schedule(self.nv, self.dev)
-            thomas, alpha = sch["kind"], sch["alpha"]
+            thomas, alpha, sweep = sch["kind"], sch["alpha"], sch["sweep"]
         self.partitioned = False
-        self.set_thomas(thomas, alpha, part_len)
+        self.set_thomas(thomas, alpha, part_len, sweep=sweep)
 
     def supports_partitioned(self) -> bool:
         return (self.lib.has("pyro_flamelet_thomas_partitioned")
                 and self.nv <= 64)
 
+    def has_sweep_forms(self) -> bool:
+        """Whether the partitioned sweep has the blocked MFMA and the
+        row-per-lane forms for this block size (16 < nv <= 63)."""
+        return (self.lib.has("pyro_flamelet_thomas_sweep_form")
+                and 16 < self.nv <= 63)
+
     def set_thomas(self, kind: str, alpha: Optional[float] = None,
-                   part_len: Optional[int] = None):
+                   part_len: Optional[int] = None, sweep: str = "mfma"):
         """Use the sequential block Thomas or the partitioned one with
         partitions of max(8, round(alpha sqrt(N))) rows (or ``part_len``);
-        the partitioned form falls back to the sequential one for nv > 64."""
+        the partitioned form falls back to the sequential one for nv > 64.
+        ``sweep`` ("mfma" or "rowlane") picks the partition sweep where the
+        block size has both (a setting of the mechanism's library)."""
         self.partitioned = kind == "partitioned" and self.supports_partitioned()
         self.part_alpha = alpha
+        if self.partitioned and self.has_sweep_forms():
+            from .autotune import SWEEPS
+            self.lib.call("pyro_flamelet_thomas_sweep_form", SWEEPS[sweep])
         if self.partitioned:
             self._setup_partitions(part_len, alpha)
 

@@ -188,6 +188,7 @@ MECH_SIGNATURES = {
                                _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                _I64, _P),
     "pyro_flamelet_thomas": (_P, _P, _P, _P, _P, _P, _I64, _P, _P, _P),
+    "pyro_flamelet_thomas_sweep_form": (_I,),
     "pyro_flamelet_thomas_partitioned": (_P, _P, _P, _P, _P, _P, _I64, _P,
                                          _P, _P, _I64, _P, _P, _P, _I64, _P,
                                          _P, _P, _P, _P, _P),

@@ -180,16 +180,18 @@ def test_gpu_badly_scaled_rows(diag, device):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mech,part_len", [("sandiego", None),
-                                           ("sandiego", 2), ("sandiego", 3),
-                                           ("sandiego", 8), ("gri30", None),
-                                           ("gri30", 5)])
-def test_gpu_partitioned_matches_oracle(mech, part_len, device):
+@pytest.mark.parametrize("mech,part_len,sweep", [
+    ("sandiego", None, "mfma"), ("sandiego", 2, "mfma"),
+    ("sandiego", 3, "mfma"), ("sandiego", 8, "mfma"),
+    ("gri30", None, "mfma"), ("gri30", 5, "mfma"), ("gri30", 2, "mfma"),
+    ("gri30", None, "rowlane"), ("gri30", 5, "rowlane")])
+def test_gpu_partitioned_matches_oracle(mech, part_len, sweep, device):
     """The partitioned flamelet block Thomas (partitions swept from both
     ends, the nv-sized recursion for the interface unknowns, recovery)
     against the oracle on ragged systems with scalar couplings, and against
     the one-wavefront kernel: nv = 10 (four sweeps per wavefront) and
-    nv = 54."""
+    nv = 54 with both sweep forms (blocked Gauss-Jordan with MFMA trailing
+    updates, and one row per lane)."""
     import torch
     from pyrometheus_amd import bundled, get_thermochem_class
     from pyrometheus_amd.flamelets.engine import FlameletBatch
@@ -203,6 +205,8 @@ def test_gpu_partitioned_matches_oracle(mech, part_len, device):
     y[-1] = 1.0
     bp = FlameletBatch(gas, grids, y, y, thomas="partitioned",
                        part_len=part_len)
+    if bp.has_sweep_forms():
+        bp.set_thomas("partitioned", None, part_len, sweep=sweep)
     bs = FlameletBatch(gas, grids, y, y, thomas="sequential")
     assert bp.partitioned and not bs.partitioned
     P = sum(sizes)
@@ -218,13 +222,19 @@ def test_gpu_partitioned_matches_oracle(mech, part_len, device):
                         uv[:, None, None] * np.eye(nv), r, o, n))
         o += n
     t = lambda a: torch.tensor(a, dtype=torch.float64, device=device)
-    for b in (bp, bs):
-        b.L.copy_(t(Lb))
-        b.U.copy_(t(Ub))
-        b.J.copy_(t(Cb))
-        b.rhs.copy_(t(Rb))
-        b.thomas()
-    xp, xs = bp.x.cpu().numpy(), bs.x.cpu().numpy()
+    print(f"[partitioned] {mech} part_len={part_len} sweep={sweep}",
+          flush=True)
+    try:
+        for b in (bp, bs):
+            b.L.copy_(t(Lb))
+            b.U.copy_(t(Ub))
+            b.J.copy_(t(Cb))
+            b.rhs.copy_(t(Rb))
+            b.thomas()
+        xp, xs = bp.x.cpu().numpy(), bs.x.cpu().numpy()
+    finally:
+        if bp.has_sweep_forms():   # the library-wide default again
+            bp.set_thomas("partitioned", None, part_len, sweep="mfma")
     for L, C, U, r, o, n in systems:
         ref = block_thomas(L, C, U, r)
         sc = np.max(np.abs(ref))
@@ -248,6 +258,7 @@ def test_gpu_autotuner_schedule(device):
                                    reps=2)
         assert res["kind"] in ("sequential", "partitioned")
         assert len(res["ms"]) == 1 + len(autotune.ALPHAS)
+        assert res["sweep"] == "mfma"   # nv = 10 has one sweep form
         assert all(t > 0.0 for t in res["ms"].values())
         best = min(res["ms"].values())
         name = (res["kind"] if res["kind"] == "sequential"
