@@ -1,7 +1,8 @@
 """pyrometheus_amd: MI355X-native thermochemistry and flamelet hot path.
 
 The generator registry mirrors ``pyrometheus/__init__.py:40-45``; this package
-registers only the ``hip`` target.  ``get_thermochem_class(mech)`` returns the
+registers the ``hip`` target (device libraries) and the ``cpp`` target (the
+reference's C++ header, host and HIP device callable).  ``get_thermochem_class(mech)`` returns the
 device class for a mechanism (the counterpart of
 ``PythonCodeGenerator.get_thermochem_class``).
 """
@@ -9,12 +10,14 @@ device class for a mechanism (the counterpart of
 import typing
 
 from .codegen import CodeGenerationOptions, CodeGenerator  # noqa: F401
+from .codegen.cpp import CppCodeGenerator
 from .codegen.hip import HipCodeGenerator
 from .mechanism import Mechanism, bundled, load_mechanism  # noqa: F401
 
 
 def get_code_generators() -> typing.Dict[str, CodeGenerator]:
-    return {HipCodeGenerator.get_name(): HipCodeGenerator}
+    return {HipCodeGenerator.get_name(): HipCodeGenerator,
+            CppCodeGenerator.get_name(): CppCodeGenerator}
 
 
 def get_thermochem_class(mech: Mechanism):

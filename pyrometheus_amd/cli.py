@@ -1,13 +1,13 @@
-"""Command line: generate the HIP source of a mechanism.
+"""Command line: generate the source of a mechanism for a target.
 
-``python -m pyrometheus_amd -l hip -m MECH.yaml -o OUT.hip -n NAME [-p PHASE]``
+``python -m pyrometheus_amd -l hip|cpp -m MECH.yaml -o OUT -n NAME [-p PHASE]``
 takes the reference's options (``pyrometheus/cli.py:11-42``: ``-l/--lang``
 from the generator registry, ``-m/--mech``, ``-o/--output``,
 ``-n/--name/--namespace``, ``-p/--phase``) and writes the generated source.
 The mechanism is read by this package's Cantera-YAML reader (Cantera itself
 is not needed).  ``--build`` additionally compiles the source for gfx950
 into a shared library next to the output (``OUT.so``) with the package's
-compiler flags.
+compiler flags (``hip`` only: the ``cpp`` target is a header).
 """
 
 from __future__ import annotations
@@ -34,7 +34,7 @@ def main(argv: Optional[Sequence[str]] = None) -> int:
     parser = argparse.ArgumentParser(
         prog="pyrometheus_amd",
         description="Code generation for combustion thermochemistry "
-                    "(MI355X HIP target).")
+                    "(MI355X HIP libraries, C++ header).")
     parser.add_argument("--version", action="version",
                         version=f"%(prog)s {_version()}")
     parser.add_argument("-l", "--lang", "--language",
@@ -59,6 +59,8 @@ def main(argv: Optional[Sequence[str]] = None) -> int:
     with open(args.output, "w") as f:
         f.write(source)
     if args.build:
+        if args.lang != "hip":
+            parser.error("--build compiles the hip target only")
         from .runtime import compile_hip
         so = os.path.splitext(args.output)[0] + ".so"
         compile_hip(os.path.abspath(args.output), so)

@@ -23,8 +23,9 @@ def _declared(header):
 
 def test_registry_and_options():
     gens = get_code_generators()
-    assert list(gens) == ["hip"]
+    assert list(gens) == ["hip", "cpp"]
     assert gens["hip"].supports_overloading()
+    assert not gens["cpp"].supports_overloading()
     with pytest.raises(TypeError):
         HipCodeGenerator.generate("x", bundled("sandiego"),
                                   CodeGenerationOptions(directive_offload="acc"))
@@ -72,6 +73,10 @@ def test_cli(tmp_path):
                  "sandiego"]) == 0
     src = out.read_text()
     assert src == HipCodeGenerator.generate("sandiego", bundled("sandiego"))
+    hpp = tmp_path / "sd.hpp"
+    assert main(["-l", "cpp", "-m", mech, "-o", str(hpp), "-n",
+                 "sandiego"]) == 0
+    assert "struct sandiego" in hpp.read_text()
     with pytest.raises(SystemExit):
         main(["-l", "python", "-m", mech, "-o", str(out), "-n", "x"])
     with pytest.raises(SystemExit):
