@@ -455,9 +455,10 @@ PYRO_API int pyro_flamelet_thomas_partitioned(
   }
 }
 
-// Sweep form of the partitioned solve for 16 < NV <= 63: 1 blocked
-// Gauss-Jordan with MFMA trailing updates (default), 0 the row-per-lane
-// eight-wave sweep; a negative form only queries.  Returns the previous
+// Form of the partitioned solve for 16 < NV <= 63: 1 the blocked
+// Gauss-Jordan sweep with MFMA trailing updates and the MFMA recursion
+// (default), 0 the row-per-lane eight-wave kernels; a negative form only
+// queries.  Returns the previous
 // form, or -3 (unsupported) when the library's NV has one form.
 PYRO_API int pyro_flamelet_thomas_sweep_form(int form) {
   if constexpr (pyro_fl::NV <= 16 || pyro_fl::NV > 63) {

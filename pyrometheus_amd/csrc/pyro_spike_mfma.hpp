@@ -396,8 +396,216 @@ k_spike_sweep_mfma(const double* __restrict__ L, const double* __restrict__ C,
   if (threadIdx.x == 0 && sh.singular && n_singular) atomicAdd(n_singular, 1);
 }
 
-// Sweep form for 16 < NV <= 63: 1 the blocked MFMA sweep above (default),
-// 0 the row-per-lane sweep of k_spike_sweep_waves.
+// k_spike_reduced_waves with its three nv x nv products on the FP64 matrix
+// cores: per partition q of a system,
+//   A: [Q | .] = B'_q [G_{q-1} | beta_{q-1}] and [P | .] = B_q [G | beta]
+//      (16 output tiles each, 14 k-steps of v_mfma_f64_16x16x4f64; B'_q and
+//      B_q staged in LDS as the A operands, G read from LDS as the B
+//      operand), then the Gauss-Jordan system [I - Q | D'_q | a' + B' beta]
+//      row by row as before (gj_waves, eight waves);
+//   C: [G_q | beta_q] = [D_q | a_q + B beta] + P [H_q | tau_q] (P from LDS
+//      as the A operand, H from LDS as the B operand).
+// The same recursion as k_spike_reduced_waves up to rounding.
+template <int NV>
+__global__ void __launch_bounds__(64 * kWaves)
+k_spike_reduced_mfma(const double* __restrict__ maps,
+                     const int64_t* __restrict__ roff,
+                     double* __restrict__ red, double* __restrict__ Z,
+                     int* __restrict__ n_singular) {
+  static_assert(NV > 16 && NV <= 64, "rows on lanes, 64-row tiles");
+  constexpr int T = Tiles<NV>::T;
+  constexpr int LD = Tiles<NV>::LD;
+  constexpr int XC = Tiles<NV>::XC;
+  static_assert(XC < 64, "beta and tau sit in the fourth column tile");
+  constexpr int MW = 2 * NV + 1;
+  constexpr int N2 = 2 * NV;
+  constexpr int RW = 2 * NV + 2;
+  constexpr int KS = (NV + 3) / 4;  // k-steps of 4
+  constexpr int LB = NV + 1;        // LDS row stride of B' and B (odd)
+  __shared__ double Gs[NV * LD], Ps[NV * LD], Hs[NV * LD];
+  __shared__ double Bp[NV * LB], Bm[NV * LB];
+  __shared__ GjShared sh;
+  const int sy = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int g = lane >> 4, cl = lane & 15;
+  const bool active = lane < NV;
+  const int r = active ? lane : 0;
+  const int64_t j0 = roff[sy], j1 = roff[sy + 1];
+  const int64_t nq = j1 - j0 + 1;
+  if (nq < 2) return;  // the whole workgroup leaves
+  const int64_t q0 = j0 + sy;
+  auto mrow = [&](int64_t q, int dir, int row) {
+    return maps + ((q * 2 + dir) * NV + row) * MW;
+  };
+  auto rrow = [&](int64_t q, int row) { return red + (q * NV + row) * RW; };
+  auto col = [&](int t) { return kWaves * t + wv; };
+  // the product tiles of this wave: column tile ct, row tiles rt0, rt0 + 1
+  const int ct = wv & 3;
+  const int rt0 = 2 * (wv >> 2);
+  const int tc = 16 * ct + cl;  // this lane's column in the product tiles
+  for (int idx = threadIdx.x; idx < NV * LD; idx += 64 * kWaves) {
+    Gs[idx] = 0.0;
+    Hs[idx] = 0.0;
+    Ps[idx] = 0.0;
+  }
+  if (threadIdx.x == 0) sh.singular = 0;
+  __syncthreads();
+  {
+    const double* md = mrow(q0, 0, r);
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int c = col(t);
+        if (c < NV) {
+          const double v = md[1 + NV + c];
+          Gs[r * LD + c] = v;
+          rrow(q0, r)[1 + c] = v;
+        }
+      }
+      if (wv == 0) {
+        Gs[r * LD + XC] = md[0];
+        rrow(q0, r)[0] = md[0];
+      }
+    }
+  }
+  __syncthreads();
+  for (int64_t q = q0 + 1; q < q0 + nq; ++q) {
+    // ---- stage B'_q and B_q (rows of the up and down maps) in LDS --------
+    for (int idx = threadIdx.x; idx < NV * NV; idx += 64 * kWaves) {
+      const int row = idx / NV, k = idx - row * NV;
+      Bp[row * LB + k] = mrow(q, 1, row)[1 + k];
+      Bm[row * LB + k] = mrow(q, 0, row)[1 + k];
+    }
+    __syncthreads();
+    // ---- A: Q = B' [G | beta], P = B [G | beta] on the matrix cores -------
+    {
+      d4 qa[2] = {}, pa[2] = {};
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int k = 4 * ks + g;
+        const double b = (k < NV && tc <= XC) ? Gs[k * LD + tc] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int i = 16 * (rt0 + u) + cl;
+          const bool ok = i < NV && k < NV;
+          const double ap = ok ? Bp[i * LB + k] : 0.0;
+          const double am = ok ? Bm[i * LB + k] : 0.0;
+          qa[u] = mfma_f64(ap, b, qa[u]);
+          pa[u] = mfma_f64(am, b, pa[u]);
+        }
+      }
+      // Q to Hs (H of the previous partition is no longer needed), P and
+      // a + B beta to Ps
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const int row = 16 * (rt0 + u) + g + 4 * jj;
+          if (row < NV && (tc < NV || tc == XC)) {
+            Hs[row * LD + tc] = qa[u][jj];
+            Ps[row * LD + tc] =
+                tc == XC ? mrow(q, 0, row)[0] + pa[u][jj] : pa[u][jj];
+          }
+        }
+    }
+    __syncthreads();
+    double a0[T], ab[1][T];
+    double x = mrow(q, 1, r)[0];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const int c = col(t);
+      a0[t] = (active && c < NV) ? ((c == r) ? 1.0 : 0.0) - Hs[r * LD + c]
+                                 : 0.0;
+      ab[0][t] = (active && c < NV) ? mrow(q, 1, r)[1 + NV + c] : 0.0;
+    }
+    x = (wv == 0 && active) ? x + Hs[r * LD + XC] : 0.0;
+    __syncthreads();  // every wave has read Q before H overwrites it
+    int my_k;
+    gj_waves<NV, T, 1>(a0, ab, x, lane, wv, my_k, sh);
+    if (active) {
+      double* o = rrow(q, my_k);
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int c = col(t);
+        if (c < NV) {
+          Hs[my_k * LD + c] = ab[0][t];
+          o[2 + NV + c] = ab[0][t];
+        }
+      }
+      if (wv == 0) {
+        Hs[my_k * LD + XC] = x;
+        o[1 + NV] = x;
+      }
+    }
+    __syncthreads();
+    // ---- C: [G_q | beta_q] = [D | a + B beta] + P [H | tau] -------------
+    {
+      d4 ga[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const int row = 16 * (rt0 + u) + g + 4 * jj;
+          double v = 0.0;
+          if (row < NV) {
+            if (tc < NV) v = mrow(q, 0, row)[1 + NV + tc];
+            if (tc == XC) v = Ps[row * LD + XC];
+          }
+          ga[u][jj] = v;
+        }
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int k = 4 * ks + g;
+        const double b = (k < NV && tc <= XC) ? Hs[k * LD + tc] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int i = 16 * (rt0 + u) + cl;
+          const double a = (i < NV && k < NV) ? Ps[i * LD + k] : 0.0;
+          ga[u] = mfma_f64(a, b, ga[u]);
+        }
+      }
+      __syncthreads();  // every wave has read G_{q-1}'s products (P, H)
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const int row = 16 * (rt0 + u) + g + 4 * jj;
+          if (row < NV && (tc < NV || tc == XC)) {
+            Gs[row * LD + tc] = ga[u][jj];
+            rrow(q, row)[tc == XC ? 0 : 1 + tc] = ga[u][jj];
+          }
+        }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && sh.singular && n_singular) atomicAdd(n_singular, 1);
+  if (wv != 0) return;
+  // back substitution on wave 0 (the rows of red were written by all the
+  // waves before the last barrier); t in natural row order, lane r
+  const int64_t ql = q0 + nq - 1;
+  double t = rrow(ql, r)[1 + NV];
+  for (int64_t q = ql - 1; q >= q0; --q) {
+    const double* o = rrow(q, r);
+    double b = o[0];
+#pragma unroll
+    for (int mm = 0; mm < NV; ++mm) b = fma(o[1 + mm], bcast(t, mm), b);
+    const int64_t j = j0 + (q - q0);
+    if (active) {
+      Z[j * N2 + r] = b;
+      Z[j * N2 + NV + r] = t;
+    }
+    if (q > q0) {
+      double tn = o[1 + NV];
+#pragma unroll
+      for (int mm = 0; mm < NV; ++mm) tn = fma(o[2 + NV + mm], bcast(t, mm), tn);
+      t = tn;
+    }
+  }
+}
+
+// Form for 16 < NV <= 63: 1 the blocked MFMA sweep and the MFMA recursion
+// above (default), 0 k_spike_sweep_waves and k_spike_reduced_waves.
 inline int& sweep_form() {
   static int form = 1;
   return form;
@@ -437,10 +645,21 @@ int solve(const double* L, const double* C, const double* U, const double* R,
     if constexpr (NV <= 16)
       k_spike_reduced<NV><<<static_cast<unsigned>(nsys), 64, 0, stream>>>(
           maps, roff, red, Z, n_singular);
-    else
-      k_spike_reduced_waves<NV>
-          <<<static_cast<unsigned>(nsys), 64 * kWaves, 0, stream>>>(
-              maps, roff, red, Z, n_singular);
+    else {
+      bool mfma = false;
+      if constexpr (NV <= 63) {
+        if (sweep_form() == 1) {
+          mfma = true;
+          k_spike_reduced_mfma<NV>
+              <<<static_cast<unsigned>(nsys), 64 * kWaves, 0, stream>>>(
+                  maps, roff, red, Z, n_singular);
+        }
+      }
+      if (!mfma)
+        k_spike_reduced_waves<NV>
+            <<<static_cast<unsigned>(nsys), 64 * kWaves, 0, stream>>>(
+                maps, roff, red, Z, n_singular);
+    }
   }
   k_spike_recover<NV><<<static_cast<unsigned>(nparts), 64, 0, stream>>>(
       fact, pstart, pend, jbefore, jafter, Z, X);

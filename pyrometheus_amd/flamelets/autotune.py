@@ -9,10 +9,11 @@ which alpha, depends on the block size, on the device and on the spread of
 flamelet sizes: short partitions mean short sweeps but a longer interface
 recursion.
 
-For 16 < nv <= 63 the partitioned sweep has two forms: ``"mfma"``, a
-blocked Gauss-Jordan whose trailing updates run on the FP64 matrix cores,
-and ``"rowlane"``, one logical row per lane with the pivot row broadcast by
-``v_readlane``; both are candidates.
+For 16 < nv <= 63 the partitioned solve has two forms: ``"mfma"``, a
+blocked Gauss-Jordan sweep whose trailing updates, and an interface
+recursion whose nv x nv products, run on the FP64 matrix cores, and
+``"rowlane"``, one logical row per lane with the pivot row broadcast by
+``v_readlane`` and the products as LDS broadcasts; both are candidates.
 
 The schedule is kept per (device, nv), never per batch: a flamelet's
 partitioning then depends on its own size only, so its arithmetic -- and its
