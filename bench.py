@@ -335,7 +335,8 @@ def derivatives_leg(args, device):
     import torch
     from pyrometheus_amd import bundled, get_thermochem_class
     from pyrometheus_amd.adjoint import (wdot_jvp, wdot_vjp, source_jacobian,
-                                         source_jacobian_workspace)
+                                         source_jacobian_workspace,
+                                         set_source_jacobian_form)
     mech = bundled("gri30")
     gas = get_thermochem_class(mech)(device=device)
     n = args.deriv_cells
@@ -390,12 +391,25 @@ def derivatives_leg(args, device):
     del wbar, vT, vY, out
     torch.cuda.empty_cache()
     jac = torch.empty((nv, nv, n), dtype=torch.float64, device=device)
+    # the two-pass form (per-reaction cache) first, for comparison
+    set_source_jacobian_form(gas, 0)
+    try:
+        work = source_jacobian_workspace(gas)
+        ms_J0 = timed(lambda: source_jacobian(gas, T, p, Y, None,
+                                              layout="entry", out=jac,
+                                              work=work), 3)
+    finally:
+        set_source_jacobian_form(gas, 1)
+    del work
     work = source_jacobian_workspace(gas)
     ms_J = timed(lambda: source_jacobian(gas, T, p, Y, None, layout="entry",
                                          out=jac, work=work), 3)
     bytes_cell = 8 * (nv * nv + ns + 2)
     res["source_jacobian"] = {
         "ms": ms_J, "mcells_s": n / ms_J / 1e3, "layout": "entry (nv, nv, N)",
+        "form": "one pass (k_source_jacobian_fused: each row evaluates its "
+                "reactions; no workspace)",
+        "two_pass_ms": ms_J0,
         "output_gb": 8 * nv * nv * n / 1e9,
         "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                      "bytes_per_cell": bytes_cell,

@@ -91,9 +91,11 @@ def source_jacobian(gas, temperature, pressure, mass_fractions, vd=None,
     block layout (``vmap(jacfwd(...))``, ``equations.py:124-129``).
     ``layout="entry"``: an (nv, nv, N) tensor, entry-major, which the kernel
     writes with fully coalesced stores.  ``source``, if given, an (nv, N)
-    tensor, receives S.  Cells are processed in fixed chunks through a
+    tensor, receives S.  The library's default form evaluates every row in
+    one pass and needs no workspace; its two-pass form
+    (``pyro_source_jacobian_form``) processes fixed chunks through a
     workspace of ``pyro_source_jacobian_work_doubles()`` doubles (the
-    per-reaction cache of one chunk); pass ``work`` to reuse one."""
+    per-reaction cache of one chunk): pass ``work`` to reuse one."""
     y, cs = gas._cells(mass_fractions)
     n = y.shape[1]
     t = gas._temperature_field(temperature, n, cs)
@@ -118,13 +120,30 @@ def source_jacobian(gas, temperature, pressure, mass_fractions, vd=None,
                   y.data_ptr(), y.stride(0), _ptr(v), out.data_ptr(), s_rc,
                   s_i, _ptr(source),
                   source.stride(0) if source is not None else n,
-                  work.data_ptr(), n, _stream())
+                  _ptr(work), n, _stream())
     return out
 
 
+def set_source_jacobian_form(gas, form: int, waves: int = 0) -> int:
+    """Select how ``source_jacobian`` evaluates the Jacobian in this process
+    (``pyro_source_jacobian_form``): 1 one pass, every row evaluating the
+    rate coefficients of its species' reactions (the default); 0 two passes
+    through a per-reaction cache.  ``waves`` (one pass): the waves sharing
+    each 64 cells, 1, 2, 4 or 8, or 0 for the library default; the values
+    do not depend on it.  Returns the previous form."""
+    if form not in (0, 1) or waves not in (0, 1, 2, 4, 8):
+        raise ValueError(f"unsupported source Jacobian form {form}, "
+                         f"waves {waves}")
+    return int(gas._lib.call("pyro_source_jacobian_form", int(form),
+                             int(waves)))
+
+
 def source_jacobian_workspace(gas, work: Optional[torch.Tensor] = None):
-    """The workspace ``source_jacobian`` needs (checked, or allocated)."""
+    """The workspace ``source_jacobian`` needs in the library's current form
+    (checked, or allocated); None when it needs none."""
     need = int(gas._lib.call("pyro_source_jacobian_work_doubles"))
+    if need == 0:
+        return None
     if work is None:
         return torch.empty(need, dtype=torch.float64, device=gas.device)
     if work.dtype != torch.float64 or work.numel() < need or \

@@ -199,6 +199,8 @@ class _Lowering:
         self.species_kc = (self.tuning["kc"] == "species" and all(
             balanced_integer(r) for r in mech.reactions if r.reversible))
         self.tab = None  # list of tabulated constants while emitting a table
+        # value -> index into tab, when constants are shared while emitting
+        self.tab_index = None
         # exp of the Arrhenius and Gibbs exponents (bounded arguments)
         self.fexp = "exp"
         # reciprocal in K_c^-1 (products of Gibbs factors)
@@ -216,6 +218,12 @@ class _Lowering:
         if self.tab is None or x in self._INLINE or (
                 x.is_integer() and -16 <= x <= 64):
             return lit(x)
+        if self.tab_index is not None:
+            j = self.tab_index.get(x)
+            if j is None:
+                j = self.tab_index[x] = len(self.tab)
+                self.tab.append(x)
+            return f"K[{j}]"
         self.tab.append(x)
         return f"K[{len(self.tab) - 1}]"
 
@@ -1786,6 +1794,367 @@ def _emit_source_jacobian_rows(w: _Writer, L: _Lowering):
     w("")
 
 
+def _kinv_linear(L: _Lowering, r: Reaction):
+    """ln K_c^-1 of a reversible reaction as linear forms in the Gibbs basis
+    (1 - ln T, T, T^2, T^3, T^4, 1/T, 1):
+        ln K_c^-1 = sum_j s_j g_j/RT + dnu ln(R T / p0)
+    (chem_expr.py:274-300; s_j the net stoichiometric coefficient, products
+    positive), with g/RT = a0 (1 - ln T) - a1 T/2 - a2 T^2/6 - a3 T^3/12
+    - a4 T^4/20 + a5/T - a6 of each species' range.  Returns (const,
+    {T_mid: (hi, lo)}): the coefficients of the single-range species and of
+    the dnu term, and per T_mid those of the two-range species of that T_mid
+    in its high and low range."""
+    def gcoef(a):
+        return [a[0], -a[1] / 2, -a[2] / 6, -a[3] / 12, -a[4] / 20, a[5],
+                -a[6]]
+    net: Dict[int, float] = {}
+    for sp, nu in r.products:
+        net[L.idx[sp]] = net.get(L.idx[sp], 0.0) + nu
+    for sp, nu in r.reactants:
+        net[L.idx[sp]] = net.get(L.idx[sp], 0.0) - nu
+    dnu = sum(nu for _, nu in r.products) - sum(nu for _, nu in r.reactants)
+    const = [0.0] * 7
+    const[0] -= dnu
+    const[6] += dnu * (1.0 + math.log(GAS_CONSTANT / ONE_ATM))
+    groups: Dict[float, Tuple[List[float], List[float]]] = {}
+    for k in sorted(net):
+        s, sp = net[k], L.m.species[k]
+        if s == 0.0:
+            continue
+        if tuple(sp.high) == tuple(sp.low):
+            for b, c in enumerate(gcoef(sp.high)):
+                const[b] += s * c
+        else:
+            hi, lo = groups.setdefault(sp.t_mid, ([0.0] * 7, [0.0] * 7))
+            for b, (ch, cl) in enumerate(zip(gcoef(sp.high), gcoef(sp.low))):
+                hi[b] += s * ch
+                lo[b] += s * cl
+    return const, groups
+
+
+def _emit_kinv_linear(w: _Writer, L: _Lowering, r: Reaction, out: str):
+    """Emit ``out`` = ln K_c^-1 from ``_kinv_linear``: one form per T_mid of
+    the reaction's two-range species, both ranges evaluated and pinned so
+    the range choice stays a per-lane select."""
+    basis = ["omlnT", "T", "T2", "T3", "T4", "invT"]
+
+    def poly(cs):
+        expr = L.c(cs[6])
+        for cc, b in zip(reversed(cs[:6]), reversed(basis)):
+            if cc != 0.0:
+                expr = f"fma({L.c(cc)}, {b}, {expr})"
+        return expr
+    const, groups = _kinv_linear(L, r)
+    if len(groups) == 1:
+        (t, (hi, lo)), = groups.items()
+        ch = [a + b for a, b in zip(const, hi)]
+        cl = [a + b for a, b in zip(const, lo)]
+        w(f"double lh = {poly(ch)}, ll = {poly(cl)};")
+        w('asm volatile("" : "+v"(lh), "+v"(ll));')
+        w(f"const double {out} = {L.tmid_flag[t]} ? lh : ll;")
+        return
+    w(f"double {out} = {poly(const)};")
+    for t in sorted(groups):
+        hi, lo = groups[t]
+        w(f"{{ double gh = {poly(hi)}, gl = {poly(lo)};")
+        w('  asm volatile("" : "+v"(gh), "+v"(gl));')
+        w(f"  {out} += {L.tmid_flag[t]} ? gh : gl; }}")
+
+
+def _fused_row_cost(L: _Lowering, k: int, inv) -> float:
+    """Relative work of row k+1 in the one-pass form (exponentials and
+    logarithms dominate): per reaction k_f and K_c^-1, the falloff blend,
+    and the row's stores."""
+    cost = 0.0
+    for i, _ in inv[k]:
+        r = L.m.reactions[i]
+        cost += 2.0 if r.reversible else 1.0
+        if r.kind == "falloff":
+            cost += 6.0 if r.falloff_type == "Troe" else 2.0
+    return cost + 0.1 * L.ns
+
+
+def _emit_jac_fused_gibbs(w: _Writer, L: _Lowering, gibbs: bool):
+    """The one-pass source Jacobian's per-cell Gibbs factors: for the
+    species k = rg (mod G), E_k = exp(g'_k) and F_k = exp(-g'_k) to this
+    cell's LDS columns Et, Ft (stride kJacFusedCells), g'_k = g_k/RT
+    shifted by element potentials (the shifts cancel in a balanced
+    reaction; the same exponents as ``emit_gibbs_exp``)."""
+    w(f"constexpr bool kJacFusedGibbs = {'true' if gibbs else 'false'};")
+    w("// cells per block of the one-pass kernel: the row stride of its LDS")
+    w("// tables (species-major)")
+    w("constexpr int kJacFusedCells = 64;")
+    w("__device__ __forceinline__ void jac_fused_gibbs(const double T, "
+      "const int rg, const int G, double* Et, double* Ft) {")
+    w.ind += 1
+    if gibbs:
+        w("constexpr int hs = kJacFusedCells;")
+        w("const double lnT = log(T);")
+        w("const double invT = 1.0 / T;")
+        L.emit_gibbs_prologue(w)
+        basis = ["omlnT", "T", "T2", "T3", "T4", "invT"]
+        for k, sp in enumerate(L.m.species):
+            def coeffs(c, k=k):
+                return [c[0], -c[1] / 2, -c[2] / 6, -c[3] / 12, -c[4] / 20,
+                        c[5] - L.shift_m[k], -(c[6] + L.shift_n[k])]
+
+            def poly(cs):
+                expr = L.c(cs[6])
+                for cc, b in zip(reversed(cs[:6]), reversed(basis)):
+                    expr = f"fma({L.c(cc)}, {b}, {expr})"
+                return expr
+            hi, lo = coeffs(sp.high), coeffs(sp.low)
+            w(f"if ({k} % G == rg) {{  // {sp.name}")
+            w.ind += 1
+            if hi == lo:
+                w(f"const double g = {poly(hi)};")
+            else:
+                w(f"double gh = {poly(hi)}, gl = {poly(lo)};")
+                w('asm volatile("" : "+v"(gh), "+v"(gl));')
+                w(f"const double g = {L.tmid_flag[sp.t_mid]} ? gh : gl;")
+            w(f"Et[{k} * hs] = exp(g);")
+            w(f"Ft[{k} * hs] = exp(-g);")
+            w.ind -= 1
+            w("}")
+    w.ind -= 1
+    w("}")
+    w("")
+
+
+def _emit_source_jacobian_fused(w: _Writer, L: _Lowering):
+    """One-pass source Jacobian: the same row assembly as
+    ``_emit_source_jacobian_rows``, but each row evaluates k_f, K_c^-1 and
+    their T- and [M]-derivatives for the reactions of its species from T
+    and the concentrations instead of reading them from a per-reaction
+    workspace.  K_c^-1 is a product of per-species factors E_k = exp(g'_k)
+    (products) and F_k = exp(-g'_k) (reactants) that the kernel tabulates
+    per cell in LDS (``jac_fused_gibbs``; mechanisms whose reversible
+    reactions are balanced with integer coefficients), else one exponential
+    of a linear form per reaction (``_kinv_linear``); its T-derivative takes
+    the species' h/RT from the dT table (dT_m = -(R T/cp) (h_m/RT) / W_m,
+    so h_m/RT = dT_m W_m rkT with rkT = -cp/(R T)).  Rows are split into
+    kJacFusedGroups groups of about equal work; a wave takes a contiguous
+    range of groups, chosen at run time, so one copy of the code serves any
+    number of waves per cell."""
+    m = L.m
+    ns = L.ns
+    ng = int(L.tuning.get("jac_fused_groups", 8))
+    inv = {k: [] for k in range(ns)}
+    for i, r in enumerate(m.reactions):
+        for k, nu in L.species_updates(r, "q"):
+            inv[k].append((i, nu))
+    rows = list(range(L.tuning.get("jac_rows", ns)))
+    cost = {k: _fused_row_cost(L, k, inv) for k in rows}
+    load = [0.0] * ng
+    load[0] = 0.1 * ns
+    group_of = {}
+    for k in sorted(rows, key=lambda k: -cost[k]):
+        g = min(range(ng), key=lambda g: load[g])
+        group_of[k] = g
+        load[g] += cost[k]
+    pin_n = int(L.tuning.get("jac_fused_pin", 2))
+    gibbs = L.species_kc
+    _emit_jac_fused_gibbs(w, L, gibbs)
+    table_at = len(w.lines)
+    w(f"constexpr int kJacFusedGroups = {ng};")
+    w("// rows of the source Jacobian in one pass: the row groups g0..g1-1;")
+    w("// Cp0, dTl0, Ep0 and Fp0 point at this lane's cell in the LDS tables")
+    w("// of C, dT, E and F (the last two unused without kJacFusedGibbs)")
+    w("__device__ __forceinline__ void source_jacobian_fused("
+      "int g0, int g1, pyro::lptr Cp0, const double Ctot0, "
+      "const double rho, const double T0, const double cp, "
+      "const double mmw, const double vd, pyro::lptr dTl0, pyro::lptr Ep0, "
+      "pyro::lptr Fp0, pyro::gmptr o0, const int64_t s_rc, pyro::gmptr S0, "
+      "const int64_t ldS) {")
+    w.ind += 1
+    w("constexpr int hs = kJacFusedCells;")
+    # one table for every row: a reaction met again in another row reuses
+    # its constants
+    L.tab, L.tab_index = [], {}
+    w("pyro::cptr K0 = (pyro::cptr)kTabJacFused;")
+    w("pyro::cptr IW0 = (pyro::cptr)kInvWMem;")
+    w("const double irho0 = 1.0 / rho;")
+    w("const double rT0 = rho * (1.0 / T0);")
+    w("const double dT00 = 1.0 / cp;")
+    w("const double dr00 = -rT0 * dT00;")
+    w("const double rm0 = -rho * mmw;  // drho_m = -rT dT_m + rm / W_m")
+    w("const double lnT0 = log(T0);")
+    w("const double invT0 = 1.0 / T0;")
+    w("const double omlnT0 = 1.0 - lnT0;")
+    w("const double T20 = T0 * T0, T30 = T20 * T0, T40 = T30 * T0;")
+    w("const double rkT0 = -cp * invT0 * (1.0 / kGasConstant);")
+    w("const double rtp0 = kGasConstant * T0 / kOneAtm;")
+    w("const double prt0 = kOneAtm / (kGasConstant * T0);")
+    L._flags(w, "T0")
+    scal = ["T", "rm", "rT", "dr0", "dT0", "irho", "lnT", "invT", "omlnT",
+            "T2", "T3", "T4", "rkT", "rtp", "prt"] + (
+                ["Ctot"] if L.uses_ctot() else [])
+    ptrs = ["Cp", "dTl"] + (["Ep", "Fp"] if gibbs else [])
+
+    def pin_chunks(names, n=20):
+        for a in range(0, len(names), n):
+            ops = ", ".join(f'"+v"({x})' for x in names[a:a + n])
+            w(f'asm volatile("" : {ops});')
+
+    def row_locals():
+        # row-local copies of the per-cell scalars and of the table, output
+        # and constant pointers, passed through an empty asm: nothing of a
+        # row is computed or loaded ahead of it (and kept live meanwhile),
+        # and no value is carried from row to row
+        w("double " + ", ".join(f"{x} = {x}0" for x in scal) + ";")
+        w("pyro::lptr " + ", ".join(f"{x} = {x}0" for x in ptrs) + ";")
+        w("pyro::gmptr o = o0, S = S0;")
+        w("pyro::cptr K = K0, IW = IW0;")
+        pin_chunks(scal + ptrs + ["o", "S"])
+        w('asm volatile("" : "+s"(IW), "+s"(K));')
+        w("const pyro::LdsColumn<hs> C{Cp};")
+        if gibbs:
+            w("const pyro::LdsColumn<hs> E{Ep}, F{Fp};")
+
+    def emit_row0():
+        w("{  // row 0: d(vd/rho)")
+        w.ind += 1
+        row_locals()
+        w("const double v0 = -vd * irho * irho;")
+        w("o[0] = v0 * dr0;")
+        w("#pragma unroll")
+        w("for (int q = 0; q < NS; ++q) {")
+        w("  const double dT = dTl[q * hs];")
+        w("  o[(1 + q) * s_rc] = v0 * fma(-rT, dT, rm * IW[q]);")
+        w("}")
+        w("if (S) S[0] = vd * irho;")
+        w.ind -= 1
+        w("}")
+
+    def emit_reaction(i, nu):
+        r = m.reactions[i]
+        nl = lit(nu)
+        w(f"{{  // R{i}: {r.equation}")
+        w.ind += 1
+        L.kf_block_deriv(w, r)
+        rpairs = L.reactant_pairs(r)
+        w(f"const double pf = {L.prod_expr(rpairs)};")
+        dpf = dict(L.prod_partials(rpairs))
+        dpb = {}
+        if r.reversible:
+            ppairs = [(L.idx[sp], n2) for sp, n2 in r.products]
+            w(f"const double pb = {L.prod_expr(ppairs)};")
+            dpb = dict(L.prod_partials(ppairs))
+            if gibbs:
+                L.kinv_f = True
+                w(f"const double kinv = {L.kinv_expr(r)};")
+                L.kinv_f = False
+            else:
+                _emit_kinv_linear(w, L, r, "lk")
+                w("const double kinv = exp(lk);")
+            # sum over reactants minus products of nu h/RT, from the dT table
+            net: Dict[int, float] = {}
+            for sp, n2 in r.reactants:
+                net[L.idx[sp]] = net.get(L.idx[sp], 0.0) + n2
+            for sp, n2 in r.products:
+                net[L.idx[sp]] = net.get(L.idx[sp], 0.0) - n2
+            expr = "0.0"
+            for j in sorted(net):
+                if net[j] != 0.0:
+                    expr = (f"fma({L.c(net[j] * m.molecular_weights[j])}, "
+                            f"dTl[{j} * hs], {expr})")
+            dnu = (sum(n2 for _, n2 in r.products)
+                   - sum(n2 for _, n2 in r.reactants))
+            w(f"const double dkinvT = kinv * invT * fma(rkT, {expr}, "
+              f"{lit(dnu)});")
+            w("const double delta = fma(-kinv, pb, pf);")
+            w("const double dqT = fma(dkfT, delta, -kf * dkinvT * pb);")
+        else:
+            w("const double delta = pf;")
+            w("const double dqT = dkfT * delta;")
+        w(f"wk = fma({nl}, kf * delta, wk);")
+        w(f"bk = fma({nl}, dqT, bk);")
+        w(f"const double a = {nl} * kf;")
+        if dpb:
+            w(f"const double b = {lit(-nu)} * (kf * kinv);")
+        touched = []
+        for j in sorted(set(dpf) | set(dpb)):
+            if j in dpf:
+                w(f"G[{j}] = fma(a, {dpf[j]}, G[{j}]);")
+            if j in dpb:
+                w(f"G[{j}] = fma(b, {dpb[j]}, G[{j}]);")
+            touched.append(j)
+        if r.kind in ("three-body", "falloff"):
+            w(f"const double mq = {nl} * dkfM * delta;")
+            for j, e in L.efficiency_terms(r):
+                w(f"G[{j}] = fma({L.c(e)}, mq, G[{j}]);")
+                touched.append(j)
+            if r.default_efficiency != 0.0:
+                w(f"dm = fma({L.c(r.default_efficiency)}, mq, dm);")
+        w.ind -= 1
+        w("}")
+        return touched
+
+    def emit_row(k):
+        w(f"{{  // row {k + 1}: {m.species[k].name}")
+        w.ind += 1
+        row_locals()
+        w("double G[NS];")
+        w("#pragma unroll")
+        w("for (int j = 0; j < NS; ++j) G[j] = 0.0;")
+        w("double wk = 0.0, bk = 0.0, dm = 0.0;")
+        pending = set()
+        for n_r, (i, nu) in enumerate(inv[k]):
+            pending.update(emit_reaction(i, nu))
+            if pin_n and (n_r + 1) % pin_n == 0:
+                # at most pin_n reactions of the row interleaved: the
+                # accumulators settle and later table reads stay below
+                pin_chunks([f"G[{j}]" for j in sorted(pending)]
+                           + ["wk", "bk", "dm"] + ptrs)
+                pending.clear()
+        w("double A = 0.0;")
+        w("#pragma unroll")
+        w("for (int j = 0; j < NS; ++j) { G[j] += dm; A = fma(G[j], C[j], A); }")
+        w(f"const double f = kW[{k}] * irho;")
+        w("const double al = f * fma(-wk, irho, A * irho);")
+        w("const double be = f * bk;")
+        w(f"pyro::gmptr row = o + {(k + 1) * (ns + 1)} * s_rc;")
+        # the store addresses are formed here, not ahead of the row
+        w('asm volatile("" : "+v"(row));')
+        w("row[0] = fma(al, dr0, be * dT0);")
+        w("#pragma unroll")
+        w("for (int q = 0; q < NS; ++q) {")
+        w("  const double dT = dTl[q * hs];")
+        w("  const double dr = fma(-rT, dT, rm * IW[q]);")
+        w(f"  row[(1 + q) * s_rc] = fma(al, dr, fma(be, dT, "
+          f"kW[{k}] * IW[q] * G[q]));")
+        w("}")
+        w(f"if (S) S[{k + 1} * ldS] = f * wk;")
+        w('asm volatile("" ::: "memory");')
+        w.ind -= 1
+        w("}")
+
+    w("for (int g = g0; g < g1; ++g) {")
+    w.ind += 1
+    w("switch (g) {")
+    for g in range(ng):
+        w(f"case {g}: {{")
+        w.ind += 1
+        if g == 0:
+            emit_row0()
+        for k in sorted(r for r in rows if group_of[r] == g):
+            emit_row(k)
+        w("break;")
+        w.ind -= 1
+        w("}")
+    w("default: break;")
+    w("}")
+    w.ind -= 1
+    w("}")
+    w.ind -= 1
+    w("}")
+    w("")
+    tab = L.tab or [0.0]
+    w.lines.insert(table_at, "__constant__ double kTabJacFused[] = {"
+                   + ", ".join(lit(x) for x in tab) + "};")
+    L.tab, L.tab_index = None, None
+
+
 def _emit_reaction_quantities(w: _Writer, L: _Lowering):
     m = L.m
     w("// per-reaction k_f, -ln K_c, r_f, r_r, r_net to memory "
@@ -2016,6 +2385,7 @@ class HipCodeGenerator(CodeGenerator):
         _emit_wdot_vjp_lazy(w, L)
         _emit_reaction_cache(w, L)
         _emit_source_jacobian_rows(w, L)
+        _emit_source_jacobian_fused(w, L)
         dense = mech.n_species <= DENSE_JAC_MAX_SPECIES
         if dense:
             _emit_wdot_jac(w, L)

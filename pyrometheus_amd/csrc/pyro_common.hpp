@@ -63,6 +63,18 @@ typedef __attribute__((address_space(1))) double* gmptr;
 // Pointer into LDS (loads through it stay ds_read rather than flat).
 typedef const __attribute__((address_space(3))) double* lptr;
 
+// One cell's column of a species-major LDS table with S cells per row:
+// entry k at p[k * S], one ds_read with an immediate offset.  It refers to
+// the pointer variable, so an asm statement that passes that variable on
+// ("+v") keeps later reads below it.
+template <int S>
+struct LdsColumn {
+  const lptr& p;
+  __device__ __forceinline__ double operator[](int k) const {
+    return p[k * S];
+  }
+};
+
 // Reciprocal from the hardware estimate (relative error below 5e-8 on
 // gfx950, measured) and one Newton step: relative error below 2.3e-15.
 __device__ __forceinline__ double recip1(double x) {

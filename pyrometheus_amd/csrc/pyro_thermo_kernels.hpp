@@ -331,6 +331,96 @@ k_source_jacobian_rows(const double* __restrict__ T,
         out + i * s_i, s_rc, S ? S + i : nullptr, ldS);
 }
 
+// One-pass row kernel, no workspace: a block of 64 G threads (G waves,
+// G dividing mech::kJacFusedGroups) takes 64 cells; its waves share the
+// cells' tables in LDS (species-major, 64 cells per row: concentrations C,
+// dT of each column, and with mech::kJacFusedGibbs the Gibbs factors
+// E = exp(g'), F = exp(-g')), each wave filling the species k = rg
+// (mod G), and wave rg then builds the row groups [rg P, (rg + 1) P),
+// P = kJacFusedGroups / G, evaluating each row's reactions itself.  The
+// mixture sums sum_k Y_k/W_k and sum_k Y_k c_p,k/W_k are taken over eight
+// fixed blocks of species (k mod 8) and the blocks added in order, so no
+// value depends on G.
+constexpr int kJacFusedCells = mech::kJacFusedCells;
+constexpr int kJacFusedTables = mech::kJacFusedGibbs ? 4 : 2;
+
+__global__ void __launch_bounds__(64 * 8)
+k_source_jacobian_fused(const double* __restrict__ T,
+                        const double* __restrict__ p, int64_t p_stride,
+                        const double* __restrict__ Y, int64_t ldY,
+                        const double* __restrict__ vd,
+                        double* __restrict__ out, int64_t s_rc, int64_t s_i,
+                        double* __restrict__ S, int64_t ldS, int64_t n) {
+  constexpr int kCells = kJacFusedCells;
+  constexpr int kTab = mech::NS * kCells;
+  __shared__ double tab[kJacFusedTables * kTab];
+  __shared__ double part[2 * 8 * kCells];
+  double* Cl = tab;
+  double* dTl = tab + kTab;
+  double* El = tab + (kJacFusedTables > 2 ? 2 * kTab : 0);
+  double* Fl = tab + (kJacFusedTables > 2 ? 3 * kTab : 0);
+  const int cl = threadIdx.x % kCells;
+  const int G = __builtin_amdgcn_readfirstlane(blockDim.x / kCells);
+  const int rg = __builtin_amdgcn_readfirstlane(threadIdx.x / kCells);
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kCells + cl;
+  // lanes past the end work on the last cell (no early return before the
+  // barriers) and store nothing
+  const int64_t i = i0 < n ? i0 : n - 1;
+  const double t = T[i];
+  {
+    // this wave's blocks b = rg (mod G) of sum Y/W and sum Y c_p/R / W
+    double cpr[mech::NS];
+    mech::species_cp_r(t, cpr);
+    double sy[8] = {}, sc[8] = {};
+#pragma unroll
+    for (int k = 0; k < mech::NS; ++k)
+      if ((k & 7) % G == rg) {
+        const double yw = Y[k * ldY + i] * mech::kInvW[k];
+        sy[k & 7] += yw;
+        sc[k & 7] = fma(yw, cpr[k], sc[k & 7]);
+      }
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      if (b % G == rg) {
+        part[b * kCells + cl] = sy[b];
+        part[(8 + b) * kCells + cl] = sc[b];
+      }
+  }
+  __syncthreads();
+  double sy = 0.0, sc = 0.0;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    sy += part[b * kCells + cl];
+    sc += part[(8 + b) * kCells + cl];
+  }
+  const double mmw = 1.0 / sy;
+  const double rho = p[i * p_stride] * mmw / (kGasConstant * t);
+  const double cp = kGasConstant * sc;
+  {
+    double hrt[mech::NS];
+    mech::species_h_rt(t, 1.0 / t, hrt);
+    const double kT = -kGasConstant * t / cp;
+#pragma unroll
+    for (int k = 0; k < mech::NS; ++k)
+      if (k % G == rg) {
+        Cl[k * kCells + cl] = mech::kInvW[k] * rho * Y[k * ldY + i];
+        dTl[k * kCells + cl] = kT * hrt[k] * mech::kInvW[k];
+      }
+    mech::jac_fused_gibbs(t, rg, G, El + cl, Fl + cl);
+  }
+  __syncthreads();
+  if (i0 >= n) return;
+  double ctot = 0.0;
+#pragma unroll
+  for (int k = 0; k < mech::NS; ++k) ctot += Cl[k * kCells + cl];
+  const int per = mech::kJacFusedGroups / G;
+  mech::source_jacobian_fused(
+      rg * per, (rg + 1) * per, (pyro::lptr)(Cl + cl), ctot, rho, t, cp, mmw,
+      vd ? vd[i] : 0.0, (pyro::lptr)(dTl + cl), (pyro::lptr)(El + cl),
+      (pyro::lptr)(Fl + cl), (pyro::gmptr)(out + i * s_i), s_rc,
+      S ? (pyro::gmptr)(S + i) : nullptr, ldS);
+}
+
 // ---------------------------------------------------------------------------
 // temperature: per-lane Newton on e(T,Y)=e* (do_energy) or h(T,Y)=h*
 // (codegen/python.py:375-396; flamelets/make_pyro.py:255-299).
@@ -573,8 +663,34 @@ PYRO_API int pyro_wdot_vjp(const double* rho, const double* T, const double* Y,
 // the split form measured slower).
 constexpr int64_t kJacSplitCells = 512 * 64;
 
+// Form of pyro_source_jacobian: 1 one pass (k_source_jacobian_fused, no
+// workspace; default), 0 the per-reaction cache and its row pass; and the
+// waves per 64 cells of the one-pass form (0: kJacFusedGroups, one row
+// group per wave; fewer waves share the four LDS tables among fewer
+// waves per compute unit).
+struct JacForm {
+  int form = 1;
+  int waves = 0;
+};
+inline JacForm& jac_form() {
+  static JacForm f;
+  return f;
+}
+
 PYRO_API int64_t pyro_source_jacobian_work_doubles(void) {
+  if (jac_form().form == 1) return 0;
   return static_cast<int64_t>(mech::kCacheSlots) * mech::kCacheLd;
+}
+
+PYRO_API int pyro_source_jacobian_form(int form, int waves) {
+  if (form < 0) return jac_form().form;
+  if (form > 1 || waves < 0 || waves > 8 ||
+      (waves && mech::kJacFusedGroups % waves != 0))
+    return pyro::kInvalidArgument;
+  const int prev = jac_form().form;
+  jac_form().form = form;
+  jac_form().waves = waves;
+  return prev;
 }
 
 PYRO_API int pyro_source_jacobian(const double* T, const double* p,
@@ -585,9 +701,20 @@ PYRO_API int pyro_source_jacobian(const double* T, const double* p,
                                   hipStream_t stream) {
   using namespace pyro_kern;
   if (n == 0) return pyro::kOk;
+  const bool fused = jac_form().form == 1;
   if (n < 0 || ldY < n || !T || !p || !Y || !out || s_rc <= 0 || s_i <= 0 ||
-      (S && ldS < n) || !work)
+      (S && ldS < n) || (!work && !fused))
     return pyro::kInvalidArgument;
+  if (fused) {
+    int g = jac_form().waves;
+    if (g == 0) g = mech::kJacFusedGroups;
+    const int64_t blocks = (n + kJacFusedCells - 1) / kJacFusedCells;
+    if (blocks > 0x7fffffff) return pyro::kInvalidArgument;
+    k_source_jacobian_fused<<<static_cast<unsigned>(blocks),
+                              kJacFusedCells * g, 0, stream>>>(
+        T, p, p_stride, Y, ldY, vd, out, s_rc, s_i, S, ldS, n);
+    return pyro::launch_status();
+  }
   constexpr int64_t chunk = mech::kCacheLd;
   for (int64_t c0 = 0; c0 < n; c0 += chunk) {
     const int64_t nc = n - c0 < chunk ? n - c0 : chunk;

@@ -10,7 +10,8 @@ launch sequence:
    (nv x nv) source Jacobian (chain rule through the analytic kinetics
    Jacobian held in registers; mechanisms with at most 16 species), or, for
    larger mechanisms, ``pyro_temperature`` for T(h, Y) followed by
-   ``pyro_source_jacobian`` (per-reaction cache and row assembly);
+   ``pyro_source_jacobian`` (each row evaluated in one pass from T and
+   the concentrations);
 2. ``pyro_flamelet_assemble`` -- residual and block-tridiagonal Jacobian,
    steady or Crank-Nicolson per flamelet;
 3. ``pyro_flamelet_thomas_partitioned`` -- block Thomas with the Laplacian
@@ -279,15 +280,16 @@ class FlameletBatch:
         # converge gets T = NaN, so its flamelet's update is not finite
         self.T.masked_fill_(self._iters < 0, float("nan"))
         if jac:
-            if self._jac_work is None:
-                need = int(self.lib.call("pyro_source_jacobian_work_doubles"))
+            need = int(self.lib.call("pyro_source_jacobian_work_doubles"))
+            if need and (self._jac_work is None
+                         or self._jac_work.numel() < need):
                 self._jac_work = torch.empty(need, dtype=torch.float64,
                                              device=self.dev)
             nv = self.nv
             self._call("pyro_source_jacobian", self.T.data_ptr(),
                        p.data_ptr(), 1, y_ptr, ldp, _ptr(vd),
                        self.J.data_ptr(), 1, nv * nv, self.S.data_ptr(), P,
-                       self._jac_work.data_ptr(), P, _stream())
+                       _ptr(self._jac_work) if need else None, P, _stream())
         else:
             # S alone: rho = p W/(R T), then W_k w_k / rho
             rho = torch.empty(P, dtype=torch.float64, device=self.dev)

@@ -182,6 +182,7 @@ MECH_SIGNATURES = {
     "pyro_source_jacobian": (_P, _P, _I64, _P, _I64, _P, _P, _I64, _I64, _P,
                              _I64, _P, _I64, _P),
     "pyro_source_jacobian_work_doubles": (),
+    "pyro_source_jacobian_form": (_I, _I),
     "pyro_flamelet_source": (_P, _I64, _P, _P, _I64, _P, _I64, _P, _I64, _P,
                              _P, _I64, _D, _I, _P, _I, _P),
     "pyro_flamelet_assemble": (_P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P,

@@ -245,12 +245,17 @@ def test_autograd_function(device):
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["sandiego", "gri30"])
 def test_source_jacobian_forms_agree(name, device):
-    """The row pass has two forms: one thread per cell builds every row (a
-    chunk of more than 32,768 cells) or the waves of a block split each
-    cell's rows (a small chunk).  Both must give the same values bit for
-    bit: the first cells of a 40,000-cell call against a 67-cell call."""
+    """The one-pass form splits each 64 cells' rows over 1, 2, 4 or 8 waves
+    (by default 4 above 32,768 cells, 8 below): every split gives the same
+    values bit for bit, also a 67-cell call against the first cells of a
+    40,000-cell call.  The two-pass form (per-reaction cache; one thread
+    per cell above 32,768 cells, the waves of a block splitting the rows
+    below) is bit for bit alike across its own forms too, and agrees with
+    the one pass to rounding (K_c^-1 there is a product of per-species
+    Gibbs factors, here one exponential per reaction)."""
     import torch
-    from pyrometheus_amd.adjoint import source_jacobian
+    from pyrometheus_amd.adjoint import (set_source_jacobian_form,
+                                         source_jacobian)
     m, gas, ref = setup(name, device)
     n = 40000
     print(f"[source jac forms] {name} n={n}", flush=True)
@@ -258,25 +263,53 @@ def test_source_jacobian_forms_agree(name, device):
     d = lambda a: dev(a, device)
     vd = np.random.default_rng(74).uniform(0, 1e3, n)
     Tt, pt, Yt, vt = d(T), d(p), d(Y), d(vd)
-    big = source_jacobian(gas, Tt, pt, Yt, vt, layout="entry")
     k = 67
-    small = source_jacobian(gas, Tt[:k].contiguous(), pt[:k].contiguous(),
-                            Yt[:, :k].contiguous(), vt[:k].contiguous(),
-                            layout="entry")
-    assert torch.equal(big[:, :, :k], small)
-    assert bool(torch.isfinite(big).all())
+    small_args = (Tt[:k].contiguous(), pt[:k].contiguous(),
+                  Yt[:, :k].contiguous(), vt[:k].contiguous())
+    try:
+        big = source_jacobian(gas, Tt, pt, Yt, vt, layout="entry")
+        assert bool(torch.isfinite(big).all())
+        small = source_jacobian(gas, *small_args, layout="entry")
+        assert torch.equal(big[:, :, :k], small)
+        for waves in (1, 2, 4, 8):
+            set_source_jacobian_form(gas, 1, waves)
+            again = source_jacobian(gas, *small_args, layout="entry")
+            assert torch.equal(again, small), waves
+        set_source_jacobian_form(gas, 0)
+        big0 = source_jacobian(gas, Tt, pt, Yt, vt, layout="entry")
+        small0 = source_jacobian(gas, *small_args, layout="entry")
+        assert torch.equal(big0[:, :, :k], small0)
+    finally:
+        set_source_jacobian_form(gas, 1)
+    # one pass against two: per row, relative to the row's largest entry
+    scale = big0.abs().amax(dim=1, keepdim=True) + 1e-300
+    err = float(((big - big0).abs() / scale).max())
+    print(f"  one pass vs cached: {err:.2e}", flush=True)
+    assert err <= 1e-11, err
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("form", [1, 0])
 @pytest.mark.parametrize("name", ["sandiego", "gri30"])
-def test_source_jacobian(name, device):
+def test_source_jacobian(name, form, device):
     """(ns+1)^2 source Jacobian per cell against the oracle's complex step
-    through T(h, Y) (both layouts), and, for the 9-species mechanism,
-    against the dense register Jacobian of the flamelet kernel."""
-    from pyrometheus_amd.adjoint import source_jacobian
+    through T(h, Y) (both layouts; the one-pass form 1 and the two-pass
+    form 0), and, for the 9-species mechanism, against the dense register
+    Jacobian of the flamelet kernel."""
+    from pyrometheus_amd.adjoint import (set_source_jacobian_form,
+                                         source_jacobian)
     m, gas, ref = setup(name, device)
     n = 67
-    print(f"[source jac] {name} n={n}", flush=True)
+    print(f"[source jac] {name} form={form} n={n}", flush=True)
+    set_source_jacobian_form(gas, form)
+    try:
+        _check_source_jacobian(m, gas, ref, n, device)
+    finally:
+        set_source_jacobian_form(gas, 1)
+
+
+def _check_source_jacobian(m, gas, ref, n, device):
+    from pyrometheus_amd.adjoint import source_jacobian
     rho, T, p, Y = _state(m, ref, n, seed=71)
     T = np.clip(T, 900.0, None)  # keep the oracle's T(h) Newton away from 300 K
     h = ref.get_mixture_enthalpy_mass(T, Y)
