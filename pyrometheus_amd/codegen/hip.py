@@ -137,7 +137,17 @@ class _Writer:
 #:  jac_waves -- waves per SIMD the compiler must budget registers for in
 #:           k_source_jacobian_rows (0: its own choice);
 #:  jac_fence -- an empty asm with a memory clobber every that many
-#:           reactions of a source-Jacobian row (0: none).
+#:           reactions of a source-Jacobian row (0: none);
+#:  jac_fused_groups -- row groups of the one-pass source Jacobian (the most
+#:           waves that can share 64 cells);
+#:  jac_fused_pin -- in the one-pass rows, the accumulators pass an empty asm
+#:           every that many reactions (0: never; then the scheduler
+#:           interleaves whole rows and spills: 2x slower);
+#:  jac_fused_pin_ptrs -- the LDS table pointers pass the same asm, so later
+#:           table reads stay below it (1, 2, 3 or 4 reactions per pin,
+#:           with or without: within 1 % on 2^22 GRI-3.0 cells);
+#:  jac_fused_ctable -- the one-pass rows read their constants from one
+#:           deduplicated table through scalar loads (off: literals).
 #: (GRI-3.0, 2^24 cells: 16.6 ms with the species arrays held in registers,
 #: 12.1 ms lazy at two waves per SIMD, 11.35 ms with zform.)
 DEFAULT_TUNING = {"pin": True, "pin_every": 3, "kc": "species", "ctable": True,
@@ -1953,6 +1963,7 @@ def _emit_source_jacobian_fused(w: _Writer, L: _Lowering):
         group_of[k] = g
         load[g] += cost[k]
     pin_n = int(L.tuning.get("jac_fused_pin", 2))
+    pin_ptrs = bool(L.tuning.get("jac_fused_pin_ptrs", True))
     gibbs = L.species_kc
     _emit_jac_fused_gibbs(w, L, gibbs)
     table_at = len(w.lines)
@@ -1969,8 +1980,9 @@ def _emit_source_jacobian_fused(w: _Writer, L: _Lowering):
     w.ind += 1
     w("constexpr int hs = kJacFusedCells;")
     # one table for every row: a reaction met again in another row reuses
-    # its constants
-    L.tab, L.tab_index = [], {}
+    # its constants (or, with jac_fused_ctable off, literals)
+    if L.tuning.get("jac_fused_ctable", True):
+        L.tab, L.tab_index = [], {}
     w("pyro::cptr K0 = (pyro::cptr)kTabJacFused;")
     w("pyro::cptr IW0 = (pyro::cptr)kInvWMem;")
     w("const double irho0 = 1.0 / rho;")
@@ -2105,24 +2117,31 @@ def _emit_source_jacobian_fused(w: _Writer, L: _Lowering):
                 # at most pin_n reactions of the row interleaved: the
                 # accumulators settle and later table reads stay below
                 pin_chunks([f"G[{j}]" for j in sorted(pending)]
-                           + ["wk", "bk", "dm"] + ptrs)
+                           + ["wk", "bk", "dm"]
+                           + (ptrs if pin_ptrs else []))
                 pending.clear()
+        # A = sum_j (G_j + dm) C_j with the default-efficiency term dm
+        # common to every column taken once
         w("double A = 0.0;")
         w("#pragma unroll")
-        w("for (int j = 0; j < NS; ++j) { G[j] += dm; A = fma(G[j], C[j], A); }")
+        w("for (int j = 0; j < NS; ++j) A = fma(G[j], C[j], A);")
+        if L.uses_ctot():
+            w("A = fma(dm, Ctot, A);")
         w(f"const double f = kW[{k}] * irho;")
         w("const double al = f * fma(-wk, irho, A * irho);")
         w("const double be = f * bk;")
+        # entry (1+k, 1+q) = al drho_q + be dT_q + (W_k/W_q) (G_q + dm) with
+        # drho_q = -rT dT_q + rm/W_q:  ga dT_q + (cr + W_k G_q)/W_q
+        w("const double ga = fma(-al, rT, be);")
+        w(f"const double cr = fma({L.c(m.molecular_weights[k])}, dm, al * rm);")
         w(f"pyro::gmptr row = o + {(k + 1) * (ns + 1)} * s_rc;")
         # the store addresses are formed here, not ahead of the row
         w('asm volatile("" : "+v"(row));')
         w("row[0] = fma(al, dr0, be * dT0);")
         w("#pragma unroll")
         w("for (int q = 0; q < NS; ++q) {")
-        w("  const double dT = dTl[q * hs];")
-        w("  const double dr = fma(-rT, dT, rm * IW[q]);")
-        w(f"  row[(1 + q) * s_rc] = fma(al, dr, fma(be, dT, "
-          f"kW[{k}] * IW[q] * G[q]));")
+        w(f"  const double t = fma({L.c(m.molecular_weights[k])}, G[q], cr);")
+        w("  row[(1 + q) * s_rc] = fma(ga, dTl[q * hs], IW[q] * t);")
         w("}")
         w(f"if (S) S[{k + 1} * ldS] = f * wk;")
         w('asm volatile("" ::: "memory");')
