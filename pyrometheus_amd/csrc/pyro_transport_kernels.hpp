@@ -146,29 +146,45 @@ k_mixture_transport(const double* __restrict__ p, int64_t p_stride,
     }
     out[i] = 0.5 * (s1 + 1.0 / s2);
   } else {
-    // (mmw - x_k W_k) / (p mmw (sum_a x_a / D_ak - x_k / D_kk)), or
-    // D_kk / p where that denominator is not positive
+    // (mmw - x_k W_k) / (p mmw sum_{a != k} x_a / D_ak), or D_kk / p where
+    // that sum is not positive (a pure species).  The reference forms the
+    // sum as the full sum minus the diagonal term; here it is summed
+    // directly (the same value without the cancellation; exactly zero for a
+    // pure species, as there), in the same order of a, and D_ka = D_ak
+    // lets each unordered pair's fit and reciprocal serve both rows:
+    // sum_{a != k} x_a / D_ak = (1 / T^1.5) sum_{a != k} x_a / d_ak with
+    // d_ak the fit's polynomial.
     const double pr = p[i * p_stride];
     const double t15 = sqt * t;
-    for (int k = 0; k < NS; ++k) {
-      double xs = 0.0;
+    double acc[NS];
 #pragma unroll
-      for (int a = 0; a < NS; ++a)
-        xs = fma(x[a], recip(pair_diffusivity(k, a, t15, l)), xs);
-      const double dkk = pair_diffusivity(k, k, t15, l);
-      const double xk = mole_fraction(Y, ldY, i, k, mmw);
-      // the diagonal term rounded as the sum rounded it (no fused
-      // multiply-subtract), so that a pure species' denominator is exactly
-      // zero and takes the D_kk / p branch, as in the reference
-      double diag;
-      {
-#pragma clang fp contract(off)
-        diag = xk * recip(dkk);
+    for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+    // ln T and the pair's accumulators pass an empty asm every kTrPin pairs:
+    // later pairs' fits and reciprocals are not computed ahead (and kept
+    // live beside the 2 NS per-species values)
+    constexpr int kTrPin = 4;
+    double lp = l;
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+#pragma unroll
+      for (int b = a + 1; b < NS; ++b) {
+        if ((b - a - 1) % kTrPin == 0) asm volatile("" : "+v"(lp));
+        const double r = recip(poly5((cptr)&mech::kTrDiff[a * NS + b][0], lp));
+        acc[a] = fma(x[b], r, acc[a]);
+        acc[b] = fma(x[a], r, acc[b]);
+        asm volatile("" : "+v"(acc[b]));
       }
-      const double denom = xs - diag;
+      asm volatile("" : "+v"(acc[a]));
+    }
+    const double it15 = 1.0 / t15;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const double denom = acc[k] * it15;
+      const double dkk = pair_diffusivity(k, k, t15, l);
       const double wk = ((cptr)mech::kTrW)[k];
-      out[k * ld + i] = denom > 0.0 ? (mmw - xk * wk) / (pr * mmw * denom)
-                                    : dkk / pr;
+      out[k * ld + i] = denom > 0.0
+                            ? (mmw - x[k] * wk) / (pr * mmw * denom)
+                            : dkk / pr;
     }
   }
 }
