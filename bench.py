@@ -195,6 +195,63 @@ def cpu_baseline(mech, seconds, seed, flamelet_probs=None):
     return out
 
 
+def _cpu_source_jacobian(task):
+    """Worker: the numpy oracle's (ns+1)^2 source Jacobian (complex step
+    through T(h, Y), one perturbed evaluation per column) of ``n`` cells of
+    ``name`` drawn like the workload; returns (cells, seconds)."""
+    import numpy as np
+    from oracle.flamelet_np import FlameletEquations
+    from oracle.thermochem_np import Thermochemistry
+    from pyrometheus_amd import bundled
+    from pyrometheus_amd.mechanism import ONE_ATM
+    name, seed, n = task
+    mech = bundled(name)
+    gas = Thermochemistry(mech)
+    rng = np.random.default_rng(seed)
+    T = rng.uniform(300.0, 2500.0, n)
+    p = ONE_ATM * np.exp(rng.uniform(np.log(0.5), np.log(20.0), n))
+    Y = 0.5 * rng.dirichlet(0.5 * np.ones(mech.n_species), n).T
+    Y[mech.species_index("N2")] += 0.5
+    phi = np.vstack([gas.get_mixture_enthalpy_mass(T, Y)[None], Y])
+    eq = FlameletEquations(mech, np.linspace(0.0, 1.0, n), Y[:, 0], Y[:, -1])
+    t0 = time.perf_counter()
+    eq.source_jacobian(phi, np.zeros(n), T, p)
+    return n, time.perf_counter() - t0
+
+
+def cpu_small_baselines(cores):
+    """The numpy oracle on config 1 (synth9 ω̇, 1 process and a pool over
+    ``cores``) and config 3 (the GRI-3.0 source Jacobian, 1 process)."""
+    import multiprocessing as mp
+    c1, e1 = _cpu_wdot_chunks(("synth9", 1, 4, 1 << 16))
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_cpu_wdot_chunks, [("synth9", 2000 + j, 4, 1 << 16)
+                                          for j in range(cores)])
+    wall = time.perf_counter() - t0
+    pool_cells = sum(r[0] for r in res)
+    nj, ej = _cpu_source_jacobian(("gri30", 3, 512))
+    return {
+        "config1": {
+            "value": c1 / e1 / 1e6, "unit": "Mcells/s", "cores": 1,
+            "kind": "port",
+            "sample": "4 chunks of 65,536 synth9 cells, numpy oracle",
+            "all_cores": {"value": pool_cells / wall / 1e6,
+                          "unit": "Mcells/s", "cores": cores,
+                          "sample": f"{pool_cells} cells, 4 chunks per "
+                                    f"process, {cores} processes (wall time "
+                                    "including process start)"}},
+        "source_jacobian": {
+            "value": nj / ej / 1e6, "unit": "Mcells/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{nj} gri30 cells drawn like the workload, the "
+                      "oracle's complex-step source Jacobian (one complex "
+                      "evaluation of the source terms through T(h, Y) per "
+                      "column), 1 process; the reference differentiates "
+                      "with JAX, which is absent"}}
+
+
 def _cpu_flamelet(prob):
     """Worker: the numpy oracle's solve of one benchmark flamelet (the same
     equilibration phase and hybrid solve as the device); ``prob`` is the
@@ -911,10 +968,7 @@ def main():
             fl["value_at_batch_mean_n"] = fl["value"] * f
             fl["all_cores"]["value_at_batch_mean_n"] = (
                 fl["all_cores"]["value"] * f)
-        c1, e1 = _cpu_wdot_chunks(("synth9", 1, 4, 1 << 16))
-        line["cpu_baseline"]["config1"] = {
-            "value": c1 / e1 / 1e6, "unit": "Mcells/s", "cores": 1,
-            "sample": "4 chunks of 65,536 synth9 cells, numpy oracle"}
+        line["cpu_baseline"].update(cpu_small_baselines(host_cores()))
     print(json.dumps(line), flush=True)
 
 
