@@ -103,19 +103,24 @@ class FlameletBatch:
         # jacfwd carries through it (pyro_flamelet_source, semantics 1)
         if semantics not in ("exact", "reference"):
             raise ValueError(f"unknown semantics {semantics!r}")
-        if semantics == "reference" and (jacobian == "cached" or not lib.has(
+        # "dense": the register Jacobian of pyro_flamelet_source (at most 16
+        # species); "general": pyro_source_jacobian after pyro_temperature
+        # (any mechanism; "cached" names it too, after its first form, which
+        # went through a per-reaction cache); "auto" takes the first
+        # available
+        if jacobian == "cached":
+            jacobian = "general"
+        if jacobian not in ("auto", "dense", "general"):
+            raise ValueError(f"unknown jacobian form {jacobian!r}")
+        if semantics == "reference" and (jacobian == "general" or not lib.has(
                 "pyro_flamelet_source")):
             raise HipError("reference semantics need the register-Jacobian "
                            "source kernel (at most 16 species)")
         self.semantics = semantics
-        # one-pass register Jacobian (small mechanisms) or the two-pass
-        # cached form (any mechanism); "auto" takes the first available
-        if jacobian not in ("auto", "dense", "cached"):
-            raise ValueError(f"unknown jacobian form {jacobian!r}")
         if jacobian == "dense" and not lib.has("pyro_flamelet_source"):
             raise HipError(f"{gas._mech.name}: no register-Jacobian kernel "
                            "(more than 16 species)")
-        self.dense = (jacobian != "cached"
+        self.dense = (jacobian != "general"
                       and lib.has("pyro_flamelet_source"))
         self._jac_work = None
         self._iters = None

@@ -415,7 +415,8 @@ def solve_batch(pyro_gas, problems: Sequence[FlameletProblem],
     ``update_norm``; ``kernel_events``, if a dict, collects HIP events around
     every kernel (``FlameletBatch.kernel_ms``); ``stats``, if a dict,
     receives the iteration count and the sum of batch points over them;
-    ``jacobian`` picks the source-Jacobian kernels (``FlameletBatch``);
+    ``jacobian`` picks the source-Jacobian kernels (``FlameletBatch``:
+    "dense", "general" or "auto");
     ``graphs`` replays each iteration's launches from a captured HIP graph
     (default: on a GPU unless ``kernel_events`` is given);
     ``device_control`` (default: with ``graphs``) runs every flamelet's

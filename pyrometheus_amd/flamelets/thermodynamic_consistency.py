@@ -10,7 +10,7 @@ method names, arguments, configuration schema and return values here; every
 array is a float64 tensor on the ROCm device and every step runs HIP
 kernels:
 
-* the source Jacobian (``pyro_flamelet_source`` or the cached form) turned
+* the source Jacobian (``pyro_flamelet_source`` or ``pyro_source_jacobian``) turned
   in place into the adjoint operator (``pyro_flamelet_adjoint_blocks``:
   ``equations.py:451-516``),
 * the adjoint solves through the same block-Thomas kernels as the forward
