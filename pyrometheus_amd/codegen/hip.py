@@ -117,6 +117,13 @@ class _Writer:
 #:           measured) instead of two: 1.7 % faster (the same in the falloff
 #:           terms changed the register allocation for the worse: 84 spills,
 #:           11 % slower);
+#:  ystash -- the lazy omega-dot pass keeps the mass fractions of that
+#:           many species (the last to open) in LDS from its total-
+#:           concentration pass instead of reading them from HBM again where
+#:           they open: 39 of GRI-3.0's 53 (78 KB per 256-thread block, two
+#:           blocks per CU) cut its HBM traffic from 26.8 to 22.6 GB per
+#:           2^24 cells and its time by 1 % (53 leave one block per CU: 40 %
+#:           slower);
 #:  zform -- the lazy omega-dot pass keeps F_k = exp(-g'_k) and
 #:           Z_k = C_k exp(g'_k) per species and writes a reversible rate as
 #:           q = k_f prod_r F_r (prod_r Z_r - (RT/p0)^dnu prod_p Z_p): 53
@@ -153,7 +160,7 @@ class _Writer:
 DEFAULT_TUNING = {"pin": True, "pin_every": 3, "kc": "species", "ctable": True,
                   "lazy": True, "wdot_waves": 2, "deriv_waves": 0,
                   "lazy_fence": 1, "order_tries": 8, "prefetch": 4,
-                  "zform": True, "jac_groups": 4}
+                  "zform": True, "jac_groups": 4, "ystash": 39}
 
 
 def element_shift(mech: Mechanism):
@@ -937,12 +944,40 @@ def _emit_wdot_lazy(w: _Writer, L: _Lowering):
     w("const double invT = 1.0 / T;")
     if rev:
         L.emit_gibbs_prologue(w)
+    order = reaction_order(L, L.tuning.get("order_tries", 8))
+    uses = [species_uses(L, r) for r in m.reactions]
+    # ystash: the mass fractions of that many species, the last to open,
+    # kept in LDS by the total-concentration pass for their opening (one
+    # 8-byte slot per thread of the block per species)
+    first_use = {}
+    for pos, ri in enumerate(order):
+        for k in uses[ri][0]:
+            first_use.setdefault(k, pos)
+    nst = int(L.tuning.get("ystash", 0)) if L.uses_ctot() else 0
+    stash = {k: n for n, k in enumerate(sorted(
+        first_use, key=lambda k: -first_use[k])[:nst])}
+    if stash:
+        w(f"__shared__ double ys[{len(stash)}][kWdotBlock];")
+        w("double* ysw = &ys[0][threadIdx.x];")
     if L.uses_ctot():
         # the total concentration needs every species up front; the values
-        # are loaded again where each species opens
+        # are loaded again where each species opens, from LDS for the
+        # stashed ones (each stored beside its term of the sum: as a
+        # separate pass after the sum it measured 2 % slower)
         w("double Ctot = 0.0;")
-        w("#pragma unroll")
-        w("for (int k = 0; k < NS; ++k) Ctot += kInvW[k] * rho * Y[k * ldY + i];")
+        if not stash:
+            w("#pragma unroll")
+            w("for (int k = 0; k < NS; ++k) "
+              "Ctot += kInvW[k] * rho * Y[k * ldY + i];")
+        for k in range(L.ns) if stash else ():
+            if k in stash:
+                w(f"{{ const double y = Y[{k} * ldY + i]; "
+                  f"Ctot += kInvW[{k}] * rho * y; "
+                  f"ysw[{stash[k]} * kWdotBlock] = y; }}")
+            else:
+                w(f"Ctot += kInvW[{k}] * rho * Y[{k} * ldY + i];")
+    if stash:
+        w("pyro::lptr ysr = (pyro::lptr)ysw;")
     # Yg: Y as a global-memory pointer that is passed through an empty asm
     # before each window opens species, so the optimiser cannot reuse the
     # values of the pass above or hoist the loads of later windows
@@ -956,8 +991,6 @@ def _emit_wdot_lazy(w: _Writer, L: _Lowering):
     zform = rev and L.tuning.get("zform", False)
     if zform:
         w("double F[NS], Z[NS];")
-    order = reaction_order(L, L.tuning.get("order_tries", 8))
-    uses = [species_uses(L, r) for r in m.reactions]
     last_w = {}
     for pos, ri in enumerate(order):
         for k in uses[ri][2]:
@@ -980,8 +1013,13 @@ def _emit_wdot_lazy(w: _Writer, L: _Lowering):
         c_u, e_u, w_u = uses[ri]
         if pos in opens_at:
             w('asm volatile("" : "+s"(Yg));')
+            if stash:
+                w('asm volatile("" : "+v"(ysr));')
             for k in opens_at[pos]:
-                w(f"yv[{k}] = Yg[{k} * ldY + i];")
+                if k in stash:
+                    w(f"yv[{k}] = ysr[{stash[k]} * kWdotBlock];")
+                else:
+                    w(f"yv[{k}] = Yg[{k} * ldY + i];")
         for k in sorted(c_u - open_c):
             w(f"C[{k}] = kInvW[{k}] * rho * yv[{k}];")
         for k in sorted(e_u - open_e):
