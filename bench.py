@@ -461,12 +461,17 @@ def derivatives_leg(args, device):
     work = source_jacobian_workspace(gas)
     ms_J = timed(lambda: source_jacobian(gas, T, p, Y, None, layout="entry",
                                          out=jac, work=work), 3)
+    # the flamelet solver's per-point block layout, in the same memory
+    ms_Jc = timed(lambda: source_jacobian(gas, T, p, Y, None, layout="cell",
+                                          out=jac.view(n, nv, nv),
+                                          work=work), 3)
     bytes_cell = 8 * (nv * nv + ns + 2)
     res["source_jacobian"] = {
         "ms": ms_J, "mcells_s": n / ms_J / 1e3, "layout": "entry (nv, nv, N)",
         "form": "one pass (k_source_jacobian_fused: each row evaluates its "
                 "reactions; no workspace)",
         "two_pass_ms": ms_J0,
+        "cell_layout_ms": ms_Jc,
         "output_gb": 8 * nv * nv * n / 1e9,
         "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                      "bytes_per_cell": bytes_cell,

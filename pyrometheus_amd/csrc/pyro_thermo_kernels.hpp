@@ -340,7 +340,9 @@ k_source_jacobian_rows(const double* __restrict__ T,
 // P = kJacFusedGroups / G, evaluating each row's reactions itself.  The
 // mixture sums sum_k Y_k/W_k and sum_k Y_k c_p,k/W_k are taken over eight
 // fixed blocks of species (k mod 8) and the blocks added in order, so no
-// value depends on G.
+// value depends on G.  With s_i == 1 (entry-major) a wave's store covers
+// 512 contiguous bytes and the entries are written with streaming stores;
+// per-cell blocks keep ordinary stores, which the cache merges into lines.
 constexpr int kJacFusedCells = mech::kJacFusedCells;
 constexpr int kJacFusedTables = mech::kJacFusedGibbs ? 4 : 2;
 
@@ -418,7 +420,7 @@ k_source_jacobian_fused(const double* __restrict__ T,
       rg * per, (rg + 1) * per, (pyro::lptr)(Cl + cl), ctot, rho, t, cp, mmw,
       vd ? vd[i] : 0.0, (pyro::lptr)(dTl + cl), (pyro::lptr)(El + cl),
       (pyro::lptr)(Fl + cl), (pyro::gmptr)(out + i * s_i), s_rc,
-      S ? (pyro::gmptr)(S + i) : nullptr, ldS);
+      S ? (pyro::gmptr)(S + i) : nullptr, ldS, s_i == 1);
 }
 
 // ---------------------------------------------------------------------------
