@@ -162,6 +162,10 @@ class _Writer:
 #:           keep ordinary stores: there each lane's store lands on its own
 #:           line, and streaming them made a GRI-3.0 flamelet solve about
 #:           30 % slower;
+#:  jac_stage -- per-cell blocks (each cell's row contiguous) pass each row
+#:           through a per-wave LDS buffer, 8 entries of 64 cells at a time,
+#:           so that a wave's store writes 8 cells x 64 contiguous bytes
+#:           instead of touching 64 lines;
 #:  nt_wdot -- the same for omega-dot's rates (11.23 -> 11.12 ms on 2^24
 #:           GRI-3.0 cells, timed side by side);
 #:  nt_rq -- the same for the per-reaction outputs (k_f and the like: no
@@ -2028,7 +2032,8 @@ def _emit_source_jacobian_fused(w: _Writer, L: _Lowering):
       "const double rho, const double T0, const double cp, "
       "const double mmw, const double vd, pyro::lptr dTl0, pyro::lptr Ep0, "
       "pyro::lptr Fp0, pyro::gmptr o0, const int64_t s_rc, pyro::gmptr S0, "
-      "const int64_t ldS, const bool nt) {")
+      "const int64_t ldS, const int mode, pyro::lmptr sw0, pyro::lptr sr0, "
+      "pyro::gmptr ob0, const unsigned lob, const int64_t s8) {")
     w.ind += 1
     w("constexpr int hs = kJacFusedCells;")
     # one table for every row: a reaction met again in another row reuses
@@ -2067,25 +2072,104 @@ def _emit_source_jacobian_fused(w: _Writer, L: _Lowering):
         # and no value is carried from row to row
         w("double " + ", ".join(f"{x} = {x}0" for x in scal) + ";")
         w("pyro::lptr " + ", ".join(f"{x} = {x}0" for x in ptrs) + ";")
-        w("pyro::gmptr o = o0, S = S0;")
+        w("pyro::gmptr o = o0, S = S0, ob = ob0;")
+        w("pyro::lmptr sw = sw0;")
+        w("pyro::lptr sr = sr0;")
         w("pyro::cptr K = K0, IW = IW0;")
-        pin_chunks(scal + ptrs + ["o", "S"])
-        w('asm volatile("" : "+s"(IW), "+s"(K));')
+        pin_chunks(scal + ptrs + ["o", "S", "sw", "sr"])
+        w('asm volatile("" : "+s"(IW), "+s"(K), "+s"(ob));')
         w("const pyro::LdsColumn<hs> C{Cp};")
         if gibbs:
             w("const pyro::LdsColumn<hs> E{Ep}, F{Fp};")
+
+    nvv = ns + 1
+    if nvv < 8:
+        w("const int j8 = threadIdx.x & 7;")
+    nt_jac = bool(L.tuning.get("nt_jac", True))
+    stage = bool(L.tuning.get("jac_stage", True))
+
+    def emit_stores(r, v0, vq):
+        """Store row r of this lane's cell: entry 0 = v0, entry 1 + q = vq
+        with {q} replaced.  mode 1 (consecutive cells adjacent): streaming
+        stores; mode 2 (each cell's row contiguous, a full block of 64
+        cells): through this wave's LDS buffer, 8 entries of the 64 cells
+        at a time, so that each store instruction writes 8 cells x 64
+        contiguous bytes at the scalar base ob plus this lane's byte offset
+        lob; mode 0: ordinary stores at the given strides."""
+        def loop(streaming):
+            w(f"pyro::gmptr row = o + {r * nvv} * s_rc;")
+            # the store addresses are formed here, not ahead of the row
+            w('asm volatile("" : "+v"(row));')
+            w(f"row[0] = {v0};")
+            w("#pragma unroll")
+            w("for (int q = 0; q < NS; ++q) {")
+            val = vq.format(q="q")
+            if streaming:
+                w(f"  __builtin_nontemporal_store({val}, "
+                  "&row[(1 + q) * s_rc]);")
+            else:
+                w(f"  row[(1 + q) * s_rc] = {val};")
+            w("}")
+
+        def staged():
+            # chunks of 8 entries; the last one ends at entry nvv - 1 and
+            # may overlap the one before (the same values stored twice)
+            starts = list(range(0, nvv - 7, 8))
+            if nvv % 8:
+                starts.append(max(nvv - 8, 0))
+            for e0 in starts:
+                mm = min(8, nvv - e0)
+                # the chunk's values first, then its LDS writes: a table
+                # read between two writes would wait for the earlier write
+                vals = [v0 if e == 0 else vq.format(q=e - 1)
+                        for e in range(e0, e0 + mm)]
+                w("{")
+                w("const double " + ", ".join(
+                    f"x{j} = {v}" for j, v in enumerate(vals)) + ";")
+                for j in range(mm):
+                    w(f"sw[{j}] = x{j};")
+                w("}")
+                # LDS runs in order per wave: the chunk's writes land before
+                # its reads, and its reads before the next chunk's writes
+                w('asm volatile("" ::: "memory");')
+                for pp in range(8):
+                    st = (f"pyro::store_at(ob + ({pp} * s8 + {r * nvv + e0}), "
+                          f"lob, sr[{pp * 72}]);")
+                    # lanes past the chunk's entries would store the next
+                    # entries' stale slots
+                    w(st if mm == 8 else f"if (j8 < {mm}) {st}")
+                w('asm volatile("" ::: "memory");')
+
+        def streaming():
+            # the empty asm statements open and close the streaming arm so
+            # that it shares no leading or trailing instructions with the
+            # others: merged stores would lose the non-temporal hint
+            w('asm volatile("" ::: "memory");')
+            loop(True)
+            w('asm volatile("" ::: "memory");')
+
+        arms = ([("mode == 2", staged)] if stage else []) + (
+            [("mode == 1", streaming)] if nt_jac else [])
+        for n_a, (cond, body) in enumerate(arms):
+            w(("if" if n_a == 0 else "} else if") + f" ({cond}) {{")
+            w.ind += 1
+            body()
+            w.ind -= 1
+        if arms:
+            w("} else {")
+            w.ind += 1
+        loop(False)
+        if arms:
+            w.ind -= 1
+            w("}")
 
     def emit_row0():
         w("{  // row 0: d(vd/rho)")
         w.ind += 1
         row_locals()
         w("const double v0 = -vd * irho * irho;")
-        w("o[0] = v0 * dr0;")
-        w("#pragma unroll")
-        w("for (int q = 0; q < NS; ++q) {")
-        w("  const double dT = dTl[q * hs];")
-        w("  o[(1 + q) * s_rc] = v0 * fma(-rT, dT, rm * IW[q]);")
-        w("}")
+        emit_stores(0, "v0 * dr0",
+                    "v0 * fma(-rT, dTl[{q} * hs], rm * IW[{q}])")
         w("if (S) S[0] = vd * irho;")
         w.ind -= 1
         w("}")
@@ -2186,43 +2270,10 @@ def _emit_source_jacobian_fused(w: _Writer, L: _Lowering):
         # drho_q = -rT dT_q + rm/W_q:  ga dT_q + (cr + W_k G_q)/W_q
         w("const double ga = fma(-al, rT, be);")
         w(f"const double cr = fma({L.c(m.molecular_weights[k])}, dm, al * rm);")
-        w(f"pyro::gmptr row = o + {(k + 1) * (ns + 1)} * s_rc;")
-        # the store addresses are formed here, not ahead of the row
-        w('asm volatile("" : "+v"(row));')
-        w("row[0] = fma(al, dr0, be * dT0);")
         wk_c = L.c(m.molecular_weights[k])
-
-        def store_loop(streaming):
-            w("#pragma unroll")
-            w("for (int q = 0; q < NS; ++q) {")
-            w(f"  const double t = fma({wk_c}, G[q], cr);")
-            if streaming:
-                w("  __builtin_nontemporal_store("
-                  "fma(ga, dTl[q * hs], IW[q] * t), &row[(1 + q) * s_rc]);")
-            else:
-                w("  row[(1 + q) * s_rc] = fma(ga, dTl[q * hs], IW[q] * t);")
-            w("}")
-
-        if L.tuning.get("nt_jac", True):
-            # streaming stores only where a wave's store covers whole lines
-            # (nt: consecutive cells adjacent); scattered per-cell rows go
-            # through the cache, which merges them into lines
-            # the empty asm statements open and close the streaming arm so
-            # that it shares no leading or trailing instructions with the
-            # other: merged stores would lose the non-temporal hint
-            w("if (nt) {")
-            w.ind += 1
-            w('asm volatile("" ::: "memory");')
-            store_loop(True)
-            w('asm volatile("" ::: "memory");')
-            w.ind -= 1
-            w("} else {")
-            w.ind += 1
-            store_loop(False)
-            w.ind -= 1
-            w("}")
-        else:
-            store_loop(False)
+        emit_stores(k + 1, "fma(al, dr0, be * dT0)",
+                    f"fma(ga, dTl[{{q}} * hs], IW[{{q}}] * "
+                    f"fma({wk_c}, G[{{q}}], cr))")
         w(f"if (S) S[{k + 1} * ldS] = f * wk;")
         w('asm volatile("" ::: "memory");')
         w.ind -= 1

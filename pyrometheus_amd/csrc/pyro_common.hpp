@@ -62,6 +62,14 @@ typedef const __attribute__((address_space(1))) double* gptr;
 typedef __attribute__((address_space(1))) double* gmptr;
 // Pointer into LDS (loads through it stay ds_read rather than flat).
 typedef const __attribute__((address_space(3))) double* lptr;
+// Writable pointer into LDS (stores through it stay ds_write).
+typedef __attribute__((address_space(3))) double* lmptr;
+// Store v at base + boff bytes: a uniform base and a 32-bit per-lane byte
+// offset, the form of a global store with a scalar base address.
+__device__ __forceinline__ void store_at(gmptr base, unsigned boff, double v) {
+  *reinterpret_cast<gmptr>(
+      reinterpret_cast<__attribute__((address_space(1))) char*>(base) + boff) = v;
+}
 
 // One cell's column of a species-major LDS table with S cells per row:
 // entry k at p[k * S], one ds_read with an immediate offset.  It refers to

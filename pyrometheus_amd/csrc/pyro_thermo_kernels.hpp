@@ -340,9 +340,13 @@ k_source_jacobian_rows(const double* __restrict__ T,
 // P = kJacFusedGroups / G, evaluating each row's reactions itself.  The
 // mixture sums sum_k Y_k/W_k and sum_k Y_k c_p,k/W_k are taken over eight
 // fixed blocks of species (k mod 8) and the blocks added in order, so no
-// value depends on G.  With s_i == 1 (entry-major) a wave's store covers
-// 512 contiguous bytes and the entries are written with streaming stores;
-// per-cell blocks keep ordinary stores, which the cache merges into lines.
+// value depends on G.  Stores by layout: with s_i == 1 (entry-major) a
+// wave's store covers 512 contiguous bytes and the entries are written
+// with streaming stores (mode 1); with s_rc == 1 (each cell's row
+// contiguous, as in per-cell blocks) each wave passes a row through its
+// LDS buffer of 64 cells x 9 doubles, 8 entries at a time, and stores 8
+// cells x 64 contiguous bytes per instruction (mode 2, blocks of 64 cells;
+// the last, partial block and other strides take ordinary stores, mode 0).
 constexpr int kJacFusedCells = mech::kJacFusedCells;
 constexpr int kJacFusedTables = mech::kJacFusedGibbs ? 4 : 2;
 
@@ -357,6 +361,7 @@ k_source_jacobian_fused(const double* __restrict__ T,
   constexpr int kTab = mech::NS * kCells;
   __shared__ double tab[kJacFusedTables * kTab];
   __shared__ double part[2 * 8 * kCells];
+  __shared__ double stage[8 * kCells * 9];
   double* Cl = tab;
   double* dTl = tab + kTab;
   double* El = tab + (kJacFusedTables > 2 ? 2 * kTab : 0);
@@ -412,6 +417,10 @@ k_source_jacobian_fused(const double* __restrict__ T,
   }
   __syncthreads();
   if (i0 >= n) return;
+  const int64_t c0 = static_cast<int64_t>(blockIdx.x) * kCells;
+  const int mode =
+      s_i == 1 ? 1 : ((s_rc == 1 && n - c0 >= kCells && 7 * s_i < (1 << 26)) ? 2 : 0);
+  double* stw = stage + rg * (kCells * 9);
   double ctot = 0.0;
 #pragma unroll
   for (int k = 0; k < mech::NS; ++k) ctot += Cl[k * kCells + cl];
@@ -420,7 +429,10 @@ k_source_jacobian_fused(const double* __restrict__ T,
       rg * per, (rg + 1) * per, (pyro::lptr)(Cl + cl), ctot, rho, t, cp, mmw,
       vd ? vd[i] : 0.0, (pyro::lptr)(dTl + cl), (pyro::lptr)(El + cl),
       (pyro::lptr)(Fl + cl), (pyro::gmptr)(out + i * s_i), s_rc,
-      S ? (pyro::gmptr)(S + i) : nullptr, ldS, s_i == 1);
+      S ? (pyro::gmptr)(S + i) : nullptr, ldS, mode,
+      (pyro::lmptr)(stw + cl * 9), (pyro::lptr)(stw + (cl >> 3) * 9 + (cl & 7)),
+      (pyro::gmptr)(out + c0 * s_i),
+      static_cast<unsigned>(((cl >> 3) * s_i + (cl & 7)) * 8), 8 * s_i);
 }
 
 // ---------------------------------------------------------------------------

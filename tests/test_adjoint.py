@@ -92,6 +92,28 @@ def _state(m, ref, n, seed):
     return rho, T, p, Y
 
 
+def _padded_cell_jacobian(gas, T, p, Y, vd, pad):
+    """Per-cell blocks with `pad` unused doubles after each block, through
+    the C ABI (s_rc = 1, s_i = nv^2 + pad); returns the (N, nv, nv) view."""
+    from pyrometheus_amd.thermochem import _stream
+    # the same input preparation as source_jacobian
+    y, cs = gas._cells(Y)
+    n = y.shape[1]
+    t = gas._temperature_field(T, n, cs)
+    pp, ps = gas._scalar_field(p, n, cs)
+    v = gas._temperature_field(vd, n, cs)
+    nv = gas.num_species + 1
+    out = torch.full((n, nv * nv + pad), float("nan"), dtype=torch.float64,
+                     device=y.device)
+    st = gas._lib.call("pyro_source_jacobian", t.data_ptr(), pp.data_ptr(),
+                       ps, y.data_ptr(), y.stride(0), v.data_ptr(),
+                       out.data_ptr(), 1, nv * nv + pad, None, n, None, n,
+                       _stream())
+    assert st == 0, st
+    assert bool(torch.isnan(out[:, nv * nv:]).all())
+    return out[:, :nv * nv].reshape(n, nv, nv)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", MECHS)
 def test_jvp_matches_complex_step(name, device):
@@ -271,10 +293,19 @@ def test_source_jacobian_forms_agree(name, device):
         assert bool(torch.isfinite(big).all())
         small = source_jacobian(gas, *small_args, layout="entry")
         assert torch.equal(big[:, :, :k], small)
+        # per-cell blocks: full blocks of 64 cells pass their rows through
+        # LDS, the partial last block stores directly; both give the
+        # entry-major values, as does a padded block stride
+        bigc = source_jacobian(gas, Tt, pt, Yt, vt, layout="cell")
+        assert torch.equal(bigc, big.permute(2, 0, 1))
+        assert torch.equal(_padded_cell_jacobian(gas, Tt, pt, Yt, vt, 5),
+                           bigc)
         for waves in (1, 2, 4, 8):
             set_source_jacobian_form(gas, 1, waves)
             again = source_jacobian(gas, *small_args, layout="entry")
             assert torch.equal(again, small), waves
+            again = source_jacobian(gas, *small_args, layout="cell")
+            assert torch.equal(again, small.permute(2, 0, 1)), waves
         set_source_jacobian_form(gas, 0)
         big0 = source_jacobian(gas, Tt, pt, Yt, vt, layout="entry")
         small0 = source_jacobian(gas, *small_args, layout="entry")
