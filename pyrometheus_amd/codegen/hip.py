@@ -2024,6 +2024,12 @@ def _emit_source_jacobian_fused(w: _Writer, L: _Lowering):
     _emit_jac_fused_gibbs(w, L, gibbs)
     table_at = len(w.lines)
     w(f"constexpr int kJacFusedGroups = {ng};")
+    # per-wave staging buffer of per-cell rows: [8 entries][kJacStageLd]
+    # doubles, cell c of entry j at j * kJacStageLd + c.  The wave writes
+    # with lane = cell (consecutive addresses) and reads 8 cells x 8
+    # entries per group of 32 lanes; a row length of 4 (mod 32) doubles
+    # puts those 32 reads on 32 distinct banks.
+    w("constexpr int kJacStageLd = 68;")
     w("// rows of the source Jacobian in one pass: the row groups g0..g1-1;")
     w("// Cp0, dTl0, Ep0 and Fp0 point at this lane's cell in the LDS tables")
     w("// of C, dT, E and F (the last two unused without kJacFusedGibbs)")
@@ -2127,14 +2133,14 @@ def _emit_source_jacobian_fused(w: _Writer, L: _Lowering):
                 w("const double " + ", ".join(
                     f"x{j} = {v}" for j, v in enumerate(vals)) + ";")
                 for j in range(mm):
-                    w(f"sw[{j}] = x{j};")
+                    w(f"sw[{j} * kJacStageLd] = x{j};")
                 w("}")
                 # LDS runs in order per wave: the chunk's writes land before
                 # its reads, and its reads before the next chunk's writes
                 w('asm volatile("" ::: "memory");')
                 for pp in range(8):
                     st = (f"pyro::store_at(ob + ({pp} * s8 + {r * nvv + e0}), "
-                          f"lob, sr[{pp * 72}]);")
+                          f"lob, sr[{pp * 8}]);")
                     # lanes past the chunk's entries would store the next
                     # entries' stale slots
                     w(st if mm == 8 else f"if (j8 < {mm}) {st}")

@@ -344,7 +344,7 @@ k_source_jacobian_rows(const double* __restrict__ T,
 // wave's store covers 512 contiguous bytes and the entries are written
 // with streaming stores (mode 1); with s_rc == 1 (each cell's row
 // contiguous, as in per-cell blocks) each wave passes a row through its
-// LDS buffer of 64 cells x 9 doubles, 8 entries at a time, and stores 8
+// LDS buffer of 8 entries x 64 cells, 8 entries at a time, and stores 8
 // cells x 64 contiguous bytes per instruction (mode 2, blocks of 64 cells;
 // the last, partial block and other strides take ordinary stores, mode 0).
 constexpr int kJacFusedCells = mech::kJacFusedCells;
@@ -361,7 +361,7 @@ k_source_jacobian_fused(const double* __restrict__ T,
   constexpr int kTab = mech::NS * kCells;
   __shared__ double tab[kJacFusedTables * kTab];
   __shared__ double part[2 * 8 * kCells];
-  __shared__ double stage[8 * kCells * 9];
+  __shared__ double stage[8 * 8 * mech::kJacStageLd];
   double* Cl = tab;
   double* dTl = tab + kTab;
   double* El = tab + (kJacFusedTables > 2 ? 2 * kTab : 0);
@@ -420,7 +420,7 @@ k_source_jacobian_fused(const double* __restrict__ T,
   const int64_t c0 = static_cast<int64_t>(blockIdx.x) * kCells;
   const int mode =
       s_i == 1 ? 1 : ((s_rc == 1 && n - c0 >= kCells && 7 * s_i < (1 << 26)) ? 2 : 0);
-  double* stw = stage + rg * (kCells * 9);
+  double* stw = stage + rg * (8 * mech::kJacStageLd);
   double ctot = 0.0;
 #pragma unroll
   for (int k = 0; k < mech::NS; ++k) ctot += Cl[k * kCells + cl];
@@ -430,7 +430,8 @@ k_source_jacobian_fused(const double* __restrict__ T,
       vd ? vd[i] : 0.0, (pyro::lptr)(dTl + cl), (pyro::lptr)(El + cl),
       (pyro::lptr)(Fl + cl), (pyro::gmptr)(out + i * s_i), s_rc,
       S ? (pyro::gmptr)(S + i) : nullptr, ldS, mode,
-      (pyro::lmptr)(stw + cl * 9), (pyro::lptr)(stw + (cl >> 3) * 9 + (cl & 7)),
+      (pyro::lmptr)(stw + cl),
+      (pyro::lptr)(stw + (cl & 7) * mech::kJacStageLd + (cl >> 3)),
       (pyro::gmptr)(out + c0 * s_i),
       static_cast<unsigned>(((cl >> 3) * s_i + (cl & 7)) * 8), 8 * s_i);
 }
