@@ -166,6 +166,9 @@ class _Writer:
 #:           through a per-wave LDS buffer, 8 entries of 64 cells at a time,
 #:           so that a wave's store writes 8 cells x 64 contiguous bytes
 #:           instead of touching 64 lines;
+#:  jac_stage_pipe -- those rows pipelined: each chunk is read back from LDS
+#:           while the next one is computed, and stored after that one's
+#:           LDS writes;
 #:  nt_wdot -- the same for omega-dot's rates (11.23 -> 11.12 ms on 2^24
 #:           GRI-3.0 cells, timed side by side);
 #:  nt_rq -- the same for the per-reaction outputs (k_f and the like: no
@@ -2123,10 +2126,12 @@ def _emit_source_jacobian_fused(w: _Writer, L: _Lowering):
             starts = list(range(0, nvv - 7, 8))
             if nvv % 8:
                 starts.append(max(nvv - 8, 0))
-            for e0 in starts:
-                mm = min(8, nvv - e0)
+            pipe = bool(L.tuning.get("jac_stage_pipe", True))
+
+            def put(e0):
                 # the chunk's values first, then its LDS writes: a table
                 # read between two writes would wait for the earlier write
+                mm = min(8, nvv - e0)
                 vals = [v0 if e == 0 else vq.format(q=e - 1)
                         for e in range(e0, e0 + mm)]
                 w("{")
@@ -2135,16 +2140,48 @@ def _emit_source_jacobian_fused(w: _Writer, L: _Lowering):
                 for j in range(mm):
                     w(f"sw[{j} * kJacStageLd] = x{j};")
                 w("}")
-                # LDS runs in order per wave: the chunk's writes land before
-                # its reads, and its reads before the next chunk's writes
-                w('asm volatile("" ::: "memory");')
+
+            def get():
+                w("const double " + ", ".join(
+                    f"y{pp} = sr[{pp * 8}]" for pp in range(8)) + ";")
+
+            def out(e0):
+                mm = min(8, nvv - e0)
                 for pp in range(8):
                     st = (f"pyro::store_at(ob + ({pp} * s8 + {r * nvv + e0}), "
-                          f"lob, sr[{pp * 8}]);")
+                          f"lob, y{pp});")
                     # lanes past the chunk's entries would store the next
                     # entries' stale slots
                     w(st if mm == 8 else f"if (j8 < {mm}) {st}")
-                w('asm volatile("" ::: "memory");')
+
+            # LDS runs in order per wave: a chunk's writes land before its
+            # reads, and its reads before the next chunk's writes; the
+            # empty asm statements keep the compiler to that order
+            bar = 'asm volatile("" ::: "memory");'
+            if not pipe:
+                for e0 in starts:
+                    put(e0)
+                    w(bar)
+                    w("{")
+                    get()
+                    out(e0)
+                    w("}")
+                    w(bar)
+                return
+            # pipelined: read chunk c - 1 back, compute and write chunk c
+            # while that read is in flight, then store chunk c - 1
+            put(starts[0])
+            w(bar)
+            for c in range(1, len(starts) + 1):
+                w("{")
+                get()
+                w(bar)
+                if c < len(starts):
+                    put(starts[c])
+                    w(bar)
+                out(starts[c - 1])
+                w("}")
+            w(bar)
 
         def streaming():
             # the empty asm statements open and close the streaming arm so
