@@ -2092,10 +2092,9 @@ def _emit_source_jacobian_fused(w: _Writer, L: _Lowering):
             w("const pyro::LdsColumn<hs> E{Ep}, F{Fp};")
 
     nvv = ns + 1
-    if nvv < 8:
-        w("const int j8 = threadIdx.x & 7;")
     nt_jac = bool(L.tuning.get("nt_jac", True))
-    stage = bool(L.tuning.get("jac_stage", True))
+    # rows shorter than one chunk of 8 entries keep direct stores
+    stage = bool(L.tuning.get("jac_stage", True)) and nvv >= 8
 
     def emit_stores(r, v0, vq):
         """Store row r of this lane's cell: entry 0 = v0, entry 1 + q = vq
@@ -2125,19 +2124,18 @@ def _emit_source_jacobian_fused(w: _Writer, L: _Lowering):
             # may overlap the one before (the same values stored twice)
             starts = list(range(0, nvv - 7, 8))
             if nvv % 8:
-                starts.append(max(nvv - 8, 0))
+                starts.append(nvv - 8)
             pipe = bool(L.tuning.get("jac_stage_pipe", True))
 
             def put(e0):
                 # the chunk's values first, then its LDS writes: a table
                 # read between two writes would wait for the earlier write
-                mm = min(8, nvv - e0)
                 vals = [v0 if e == 0 else vq.format(q=e - 1)
-                        for e in range(e0, e0 + mm)]
+                        for e in range(e0, e0 + 8)]
                 w("{")
                 w("const double " + ", ".join(
                     f"x{j} = {v}" for j, v in enumerate(vals)) + ";")
-                for j in range(mm):
+                for j in range(8):
                     w(f"sw[{j} * kJacStageLd] = x{j};")
                 w("}")
 
@@ -2146,13 +2144,9 @@ def _emit_source_jacobian_fused(w: _Writer, L: _Lowering):
                     f"y{pp} = sr[{pp * 8}]" for pp in range(8)) + ";")
 
             def out(e0):
-                mm = min(8, nvv - e0)
                 for pp in range(8):
-                    st = (f"pyro::store_at(ob + ({pp} * s8 + {r * nvv + e0}), "
-                          f"lob, y{pp});")
-                    # lanes past the chunk's entries would store the next
-                    # entries' stale slots
-                    w(st if mm == 8 else f"if (j8 < {mm}) {st}")
+                    w(f"pyro::store_at(ob + ({pp} * s8 + {r * nvv + e0}), "
+                      f"lob, y{pp});")
 
             # LDS runs in order per wave: a chunk's writes land before its
             # reads, and its reads before the next chunk's writes; the

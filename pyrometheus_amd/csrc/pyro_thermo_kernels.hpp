@@ -345,8 +345,9 @@ k_source_jacobian_rows(const double* __restrict__ T,
 // with streaming stores (mode 1); with s_rc == 1 (each cell's row
 // contiguous, as in per-cell blocks) each wave passes a row through its
 // LDS buffer of 8 entries x 64 cells, 8 entries at a time, and stores 8
-// cells x 64 contiguous bytes per instruction (mode 2, blocks of 64 cells;
-// the last, partial block and other strides take ordinary stores, mode 0).
+// cells x 64 contiguous bytes per instruction (mode 2, blocks of 64 cells
+// and rows of at least 8 entries; the last, partial block, shorter rows
+// and other strides take ordinary stores, mode 0).
 constexpr int kJacFusedCells = mech::kJacFusedCells;
 constexpr int kJacFusedTables = mech::kJacFusedGibbs ? 4 : 2;
 

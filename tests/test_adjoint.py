@@ -264,8 +264,10 @@ def test_autograd_function(device):
                           atol=0.0)
 
 
+# uiuc (nv = 8, no per-species Gibbs factors) stages each per-cell row as
+# one chunk of 8 entries, sandiego (nv = 10) as two overlapping chunks
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["sandiego", "gri30"])
+@pytest.mark.parametrize("name", ["sandiego", "gri30", "uiuc"])
 def test_source_jacobian_forms_agree(name, device):
     """The one-pass form splits each 64 cells' rows over 1, 2, 4 or 8 waves
     (by default 4 above 32,768 cells, 8 below): every split gives the same
@@ -321,7 +323,7 @@ def test_source_jacobian_forms_agree(name, device):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("form", [1, 0])
-@pytest.mark.parametrize("name", ["sandiego", "gri30"])
+@pytest.mark.parametrize("name", ["sandiego", "gri30", "uiuc"])
 def test_source_jacobian(name, form, device):
     """(ns+1)^2 source Jacobian per cell against the oracle's complex step
     through T(h, Y) (both layouts; the one-pass form 1 and the two-pass
