@@ -472,6 +472,8 @@ def derivatives_leg(args, device):
                 "reactions; no workspace)",
         "two_pass_ms": ms_J0,
         "cell_layout_ms": ms_Jc,
+        "cell_layout_hbm_frac": bytes_cell * n / (ms_Jc * 1e-3) / 1e9
+                                / HBM_PEAK_GBS,
         "output_gb": 8 * nv * nv * n / 1e9,
         "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                      "bytes_per_cell": bytes_cell,
