@@ -327,3 +327,29 @@ def test_temperature_no_root_raises(device):
     # the same target from the oracle: also no convergence
     with pytest.raises(RuntimeError):
         ref.get_temperature(host(target), 1300.0, Y, True)
+
+
+@pytest.mark.parametrize("name", ["sandiego", "gri30"])
+def test_mass_average_property(name, device):
+    """get_mass_average_property, sum_k Y_k p_k / W_k (codegen/python.py:
+    287-291), pinned by closed forms: p_k = W_k gives sum_k Y_k = 1, p_k = 1
+    gives 1/W of the mixture (the HIP mixture-state kernel's), both to 1e-14;
+    and, for a random p, the oracle to 1e-13 of sum_k |Y_k p_k| / W_k."""
+    m, gas, ref = setup(name, device)
+    print(f"[mass average] {name} N={N}", flush=True)
+    T, p, Y = random_states(m, N, seed=9, inert=_inert(m))
+    Y = Y / Y.sum(0, keepdims=True)
+    Yd = dev(Y, device)
+    W = np.asarray(m.molecular_weights, dtype=float)
+    ones = host(gas.get_mass_average_property(
+        Yd, dev(np.repeat(W[:, None], N, 1), device)))
+    assert np.max(np.abs(ones - 1.0)) < 1e-14
+    inv_w = host(gas.get_mass_average_property(
+        Yd, torch.ones_like(Yd)))
+    mmw = host(gas.get_mixture_molecular_weight(Yd))
+    assert np.max(np.abs(inv_w * mmw - 1.0)) < 1e-14
+    prop = np.random.default_rng(10).normal(size=Y.shape)
+    got = host(gas.get_mass_average_property(Yd, dev(prop, device)))
+    exp = ref.get_mass_average_property(Y, prop)
+    scale = ref.get_mass_average_property(Y, np.abs(prop))
+    assert np.max(np.abs(got - exp) / scale) < 1e-13
